@@ -1,0 +1,50 @@
+// Type-erased per-configuration operations behind the C ABI.  Each
+// instantiation unit (capi_inst.hip, built once per policy and key width)
+// fills a table of these; capi.hip picks the entry matching a cf_config.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "cuckoo/kernels.hpp"
+
+namespace cfabi {
+
+/// DeviceView with the word pointer erased (same fields, same meaning).
+struct AnyView {
+    void* words;
+    size_t numBuckets;
+    unsigned long long* occupied;
+    unsigned long long* evictions;
+    unsigned int* chainHist;
+    uint32_t maxEvictions;
+    uint32_t evictionPolicy;
+};
+
+struct Ops {
+    uint32_t policy, keyBytes, bitsPerTag, bucketSize, wordBytes;
+    size_t bucketBytes;
+    size_t (*numBuckets)(size_t capacity);
+    void (*hostCandidates)(size_t nb, const void* keys, size_t n, uint64_t* i1, uint64_t* i2, uint32_t* fp);
+    hipError_t (*insert)(const AnyView&, const void* keys, size_t n, uint32_t block, hipStream_t);
+    hipError_t (*insertSorted)(const AnyView&, const void* keys, size_t n, uint32_t block, hipStream_t,
+                               cuckoo::Workspace&);
+    hipError_t (*contains)(const AnyView&, const void* keys, size_t n, bool* out, uint32_t block, hipStream_t);
+    hipError_t (*remove)(const AnyView&, const void* keys, size_t n, bool* out, uint32_t block, hipStream_t);
+    hipError_t (*candidates)(const void* keys, size_t n, size_t nb, uint64_t* i1, uint64_t* i2, uint32_t* fp,
+                             uint32_t block, hipStream_t);
+    hipError_t (*packedTags)(const void* keys, size_t n, size_t nb, void* out, uint32_t block, hipStream_t);
+};
+
+}  // namespace cfabi
+
+// One table per (policy, key width); defined in capi_inst.hip.
+#define CFABI_TABLE_DECL(P, K) const cfabi::Ops* cfabi_table_##P##_##K(size_t* count)
+CFABI_TABLE_DECL(0, 4);
+CFABI_TABLE_DECL(0, 8);
+CFABI_TABLE_DECL(1, 4);
+CFABI_TABLE_DECL(1, 8);
+CFABI_TABLE_DECL(2, 4);
+CFABI_TABLE_DECL(2, 8);

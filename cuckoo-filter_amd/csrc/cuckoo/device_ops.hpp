@@ -1,0 +1,440 @@
+// Device-side cuckoo-filter algorithm for gfx950 (wave64).
+//
+// This is the per-key method of the reference (include/CuckooFilter.cuh:
+// tryRemoveAtBucket :805-848, tryInsertAtBucket :859-904, DFS eviction
+// :917-954, BFS eviction :967-1018, insert :1029-1080, contains :1088-1098,
+// remove :1106-1116), re-designed for CDNA:
+//
+//  * A bucket is read once as a register snapshot with 16-byte vector loads
+//    (global_load_dwordx4), not word by word.  Slot choice then runs on the
+//    snapshot: one SWAR mask per word, rotated so the scan still starts at the
+//    word holding slot (tag & (bucketSize-1)) and wraps, and the lowest lane of
+//    the first qualifying word wins -- the reference's order exactly.
+//  * Every write is a relaxed device-scope compare-and-swap whose expected
+//    value comes from the snapshot; a failed CAS returns the current word,
+//    which replaces the snapshot word, and the scan continues.  A stale
+//    snapshot therefore only costs a retry; it never writes a wrong value.
+//    Reading the snapshot earlier than the reference's per-word loads is one
+//    of the interleavings the reference's relaxed loads already allow.
+//  * All operations take a trivially-copyable view by value, so kernels never
+//    dereference a host object.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <type_traits>
+
+#include "bucket_policies.hpp"
+#include "hashutil.hpp"
+#include "helpers.hpp"
+
+/**
+ * @brief Eviction policy for the cuckoo filter (CuckooFilter.cuh:19-22).
+ */
+enum class EvictionPolicy {
+    BFS,  ///< Shallow breadth-first search with DFS fallback
+    DFS   ///< Pure depth-first (greedy random-walk) search; the config default
+};
+
+namespace cuckoo {
+
+/// Compile-time layout of a bucket for one configuration (CuckooFilter.cuh:226-249).
+template <typename Config>
+struct Layout {
+    using Key = typename Config::KeyType;
+    using Tag = typename Config::TagType;
+    using Word = typename Config::WordType;
+    static constexpr size_t bitsPerTag = Config::bitsPerTag;
+    static constexpr size_t bucketSize = Config::bucketSize;
+    static constexpr size_t tagsPerWord = sizeof(Word) / sizeof(Tag);
+    static constexpr size_t wordCount = bucketSize / tagsPerWord;
+    static constexpr size_t bucketBytes = wordCount * sizeof(Word);
+    static constexpr Word fpMask = static_cast<Word>((1ULL << bitsPerTag) - 1);
+
+    static_assert(bitsPerTag == 8 || bitsPerTag == 16 || bitsPerTag == 32,
+                  "The tag must be 8, 16 or 32 bits");
+    static_assert(powerOfTwo(bucketSize), "Bucket size must be a power of 2");
+    static_assert(tagsPerWord >= 1, "TagType must fit within WordType");
+    static_assert(bucketSize % tagsPerWord == 0, "bucketSize must be divisible by tagsPerWord");
+    static_assert(powerOfTwo(wordCount), "wordCount must be a power of 2");
+};
+
+/// Everything a kernel needs about one filter, passed by value.
+template <typename Config>
+struct DeviceView {
+    using Word = typename Config::WordType;
+    Word* words;                       ///< numBuckets * wordCount words, bucket-major
+    size_t numBuckets;                 ///< as CuckooFilter::numBuckets
+    unsigned long long* occupied;      ///< occupancy counter (d_numOccupied)
+    unsigned long long* evictions;     ///< eviction counter, nullptr = not counted
+    unsigned int* chainHist;           ///< per-insert eviction-chain histogram, nullptr = off
+    uint32_t maxEvictions;             ///< DFS chain bound
+    uint32_t evictionPolicy;           ///< 0 = BFS, 1 = DFS
+};
+
+namespace detail {
+
+template <typename Word>
+__device__ __forceinline__ Word atomicLoadRelaxed(const Word* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/// Relaxed device-scope CAS; returns the word's value before the operation
+/// (== expected exactly when the swap happened).
+template <typename Word>
+__device__ __forceinline__ Word casRelaxed(Word* p, Word expected, Word desired) {
+    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return expected;
+}
+
+/// Register copy of one bucket.
+template <typename Word, size_t WC>
+struct Snap {
+    Word w[WC];
+};
+
+/// Loads a whole bucket with the widest aligned vector loads it allows.
+template <typename Word, size_t WC>
+__device__ __forceinline__ Snap<Word, WC> loadBucket(const Word* p) {
+    constexpr size_t bytes = WC * sizeof(Word);
+    Snap<Word, WC> s;
+    if constexpr (bytes % 16 == 0) {
+        using V = unsigned int __attribute__((ext_vector_type(4)));
+        V v[bytes / 16];
+#pragma unroll
+        for (size_t i = 0; i < bytes / 16; ++i) v[i] = reinterpret_cast<const V*>(p)[i];
+        __builtin_memcpy(s.w, v, bytes);
+    } else if constexpr (bytes == 8) {
+        using V = unsigned int __attribute__((ext_vector_type(2)));
+        V v = *reinterpret_cast<const V*>(p);
+        __builtin_memcpy(s.w, &v, 8);
+    } else {
+        static_assert(bytes == 4, "bucket of unexpected size");
+        unsigned int v = *reinterpret_cast<const unsigned int*>(p);
+        __builtin_memcpy(s.w, &v, 4);
+    }
+    return s;
+}
+
+template <typename Word, size_t WC>
+__device__ __forceinline__ Word pick(const Snap<Word, WC>& s, uint32_t idx) {
+    Word r = s.w[0];
+#pragma unroll
+    for (uint32_t j = 1; j < WC; ++j) r = (idx == j) ? s.w[j] : r;
+    return r;
+}
+
+template <typename Word, size_t WC>
+__device__ __forceinline__ void put(Snap<Word, WC>& s, uint32_t idx, Word v) {
+#pragma unroll
+    for (uint32_t j = 0; j < WC; ++j) s.w[j] = (idx == j) ? v : s.w[j];
+}
+
+/// First word index at or after `start` (cyclically) whose bit is set in `mask`.
+template <size_t WC>
+__device__ __forceinline__ uint32_t firstFrom(uint64_t mask, uint32_t start) {
+    if constexpr (WC == 1) {
+        return 0;
+    } else {
+        constexpr uint64_t full = (WC >= 64) ? ~0ull : ((1ull << WC) - 1);
+        const uint64_t rot = start == 0 ? mask : (((mask >> start) | (mask << (WC - start))) & full);
+        return (start + lowestBit(rot)) & (WC - 1);
+    }
+}
+
+}  // namespace detail
+
+/// The per-key operations of one configuration.
+template <typename Config>
+struct FilterOps {
+    using L = Layout<Config>;
+    using Key = typename L::Key;
+    using Tag = typename L::Tag;
+    using Word = typename L::Word;
+    using Policy = typename Config::AltBucketPolicy;
+    using View = DeviceView<Config>;
+    static constexpr size_t WC = L::wordCount;
+    static constexpr size_t TPW = L::tagsPerWord;
+    static constexpr size_t BITS = L::bitsPerTag;
+    static constexpr size_t BS = L::bucketSize;
+    static constexpr Tag EMPTY = 0;
+    /// Buckets up to 128 B are handled as register snapshots; larger ones word by word.
+    static constexpr bool kSnapshot = L::bucketBytes <= 128;
+    using SnapT = detail::Snap<Word, WC>;
+
+    static __host__ __device__ __forceinline__ Tag extractTag(Word packed, size_t lane) {
+        return static_cast<Tag>((packed >> (lane * BITS)) & L::fpMask);
+    }
+
+    static __host__ __device__ __forceinline__ Word replaceTag(Word packed, size_t lane, Tag t) {
+        const size_t shift = lane * BITS;
+        return (packed & ~(static_cast<Word>(L::fpMask) << shift)) | (static_cast<Word>(t) << shift);
+    }
+
+    /// getAlternateBucketWithNewFp (CuckooFilter.cuh:388-400).
+    static __host__ __device__ __forceinline__ cuckoo::AltBucket<Tag>
+    altWithNewFp(size_t bucket, Tag fp, size_t nb) {
+        if constexpr (Policy::usesChoiceBit) {
+            return Policy::getAlternateBucketWithNewFp(bucket, fp, nb);
+        } else {
+            return {Policy::getAlternateBucket(bucket, fp, nb), fp};
+        }
+    }
+
+    static __device__ __forceinline__ Word* bucketPtr(const View& v, size_t b) {
+        return v.words + b * WC;
+    }
+
+    static __device__ __forceinline__ uint32_t startWord(Tag tag) {
+        return static_cast<uint32_t>((tag & (BS - 1)) / TPW);
+    }
+
+    // ---------------------------------------------------------------------
+    // snapshot primitives
+    // ---------------------------------------------------------------------
+
+    static __device__ __forceinline__ uint64_t emptyWords(const SnapT& s) {
+        uint64_t m = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < WC; ++j) m |= uint64_t(getZeroMask<Tag, Word>(s.w[j]) != 0) << j;
+        return m;
+    }
+
+    static __device__ __forceinline__ uint64_t matchWords(const SnapT& s, Tag tag) {
+        const Word rep = replicateTag<Tag, Word>(tag);
+        uint64_t m = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < WC; ++j) m |= uint64_t(getZeroMask<Tag, Word>(s.w[j] ^ rep) != 0) << j;
+        return m;
+    }
+
+    static __device__ __forceinline__ bool snapHas(const SnapT& s, Tag tag) {
+        const Word rep = replicateTag<Tag, Word>(tag);
+        Word acc = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < WC; ++j) acc |= getZeroMask<Tag, Word>(s.w[j] ^ rep);
+        return acc != 0;
+    }
+
+    /// tryInsertAtBucket on a snapshot the caller already holds (CuckooFilter.cuh:859-904).
+    static __device__ __forceinline__ bool tryInsertSnap(Word* bw, SnapT& s, Tag tag) {
+        const uint32_t start = startWord(tag);
+        for (;;) {
+            const uint64_t m = emptyWords(s);
+            if (m == 0) return false;
+            const uint32_t w = detail::firstFrom<WC>(m, start);
+            const Word cur = detail::pick(s, w);
+            const uint32_t lane = lowestBit(getZeroMask<Tag, Word>(cur)) / BITS;
+            const Word old = detail::casRelaxed(bw + w, cur, replaceTag(cur, lane, tag));
+            if (old == cur) return true;
+            detail::put(s, w, old);
+        }
+    }
+
+    /// tryRemoveAtBucket on a snapshot (CuckooFilter.cuh:805-848).
+    static __device__ __forceinline__ bool tryRemoveSnap(Word* bw, SnapT& s, Tag tag) {
+        const uint32_t start = startWord(tag);
+        const Word rep = replicateTag<Tag, Word>(tag);
+        for (;;) {
+            const uint64_t m = matchWords(s, tag);
+            if (m == 0) return false;
+            const uint32_t w = detail::firstFrom<WC>(m, start);
+            const Word cur = detail::pick(s, w);
+            const uint32_t lane = lowestBit(getZeroMask<Tag, Word>(cur ^ rep)) / BITS;
+            const Word old = detail::casRelaxed(bw + w, cur, replaceTag(cur, lane, EMPTY));
+            if (old == cur) return true;
+            detail::put(s, w, old);
+        }
+    }
+
+    // ---------------------------------------------------------------------
+    // bucket-level operations (snapshot for <=128 B buckets, word loop above)
+    // ---------------------------------------------------------------------
+
+    static __device__ __forceinline__ bool tryInsertAtBucket(const View& v, size_t b, Tag tag) {
+        Word* bw = bucketPtr(v, b);
+        if constexpr (kSnapshot) {
+            SnapT s = detail::loadBucket<Word, WC>(bw);
+            return tryInsertSnap(bw, s, tag);
+        } else {
+            const uint32_t start = startWord(tag);
+            for (uint32_t i = 0; i < WC; ++i) {
+                const uint32_t w = (start + i) & (WC - 1);
+                Word cur = detail::atomicLoadRelaxed(bw + w);
+                for (;;) {
+                    const Word zm = getZeroMask<Tag, Word>(cur);
+                    if (zm == 0) break;
+                    const uint32_t lane = lowestBit(zm) / BITS;
+                    const Word old = detail::casRelaxed(bw + w, cur, replaceTag(cur, lane, tag));
+                    if (old == cur) return true;
+                    cur = old;
+                }
+            }
+            return false;
+        }
+    }
+
+    static __device__ __forceinline__ bool tryRemoveAtBucket(const View& v, size_t b, Tag tag) {
+        Word* bw = bucketPtr(v, b);
+        if constexpr (kSnapshot) {
+            SnapT s = detail::loadBucket<Word, WC>(bw);
+            return tryRemoveSnap(bw, s, tag);
+        } else {
+            const uint32_t start = startWord(tag);
+            const Word rep = replicateTag<Tag, Word>(tag);
+            for (uint32_t i = 0; i < WC; ++i) {
+                const uint32_t w = (start + i) & (WC - 1);
+                Word cur = detail::atomicLoadRelaxed(bw + w);
+                for (;;) {
+                    const Word mm = getZeroMask<Tag, Word>(cur ^ rep);
+                    if (mm == 0) break;
+                    const uint32_t lane = lowestBit(mm) / BITS;
+                    const Word old = detail::casRelaxed(bw + w, cur, replaceTag(cur, lane, EMPTY));
+                    if (old == cur) return true;
+                    cur = old;
+                }
+            }
+            return false;
+        }
+    }
+
+    /// Bucket::contains (CuckooFilter.cuh:290-346): true iff some slot == tag.
+    static __device__ __forceinline__ bool bucketContains(const View& v, size_t b, Tag tag) {
+        const Word* bw = bucketPtr(v, b);
+        if constexpr (kSnapshot) {
+            return snapHas(detail::loadBucket<Word, WC>(bw), tag);
+        } else {
+            // 128-byte chunks starting at the chunk holding the start slot, with early exit.
+            constexpr uint32_t CW = 128 / sizeof(Word);
+            constexpr uint32_t NCH = WC / CW;
+            const uint32_t c0 = startWord(tag) / CW;
+            for (uint32_t i = 0; i < NCH; ++i) {
+                const uint32_t c = (c0 + i) & (NCH - 1);
+                if (detail::Snap<Word, CW> s = detail::loadBucket<Word, CW>(bw + c * CW);
+                    FilterOps::chunkHas<CW>(s, tag)) {
+                    return true;
+                }
+            }
+            return false;
+        }
+    }
+
+    template <size_t N>
+    static __device__ __forceinline__ bool chunkHas(const detail::Snap<Word, N>& s, Tag tag) {
+        const Word rep = replicateTag<Tag, Word>(tag);
+        Word acc = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < N; ++j) acc |= getZeroMask<Tag, Word>(s.w[j] ^ rep);
+        return acc != 0;
+    }
+
+    // ---------------------------------------------------------------------
+    // eviction (CuckooFilter.cuh:917-1018)
+    // ---------------------------------------------------------------------
+
+    /// insertWithEvictionDFS.  `chain` receives the number of swaps made.
+    static __device__ bool insertDFS(const View& v, Tag fp, size_t startBucket, uint32_t& chain) {
+        Tag curFp = fp;
+        size_t curB = startBucket;
+        for (uint32_t e = 0; e < v.maxEvictions; ++e) {
+            const size_t slot = (static_cast<size_t>(curFp) + static_cast<size_t>(e) * 7) & (BS - 1);
+            const uint32_t w = static_cast<uint32_t>(slot / TPW);
+            const uint32_t lane = static_cast<uint32_t>(slot & (TPW - 1));
+            Word* p = bucketPtr(v, curB) + w;
+            Word cur = detail::atomicLoadRelaxed(p);  // the CAS loop settles the real value
+            Tag evicted;
+            for (;;) {
+                evicted = extractTag(cur, lane);
+                const Word old = detail::casRelaxed(p, cur, replaceTag(cur, lane, curFp));
+                if (old == cur) break;
+                cur = old;
+            }
+            chain = e + 1;
+            const auto alt = altWithNewFp(curB, evicted, v.numBuckets);
+            curB = alt.bucket;
+            curFp = alt.fp;
+            if (tryInsertAtBucket(v, curB, curFp)) return true;
+        }
+        return false;
+    }
+
+    /// insertWithEvictionBFS: up to min(8, bucketSize/2) candidates at slots
+    /// (fp + i*7) & (bucketSize-1), then DFS.
+    static __device__ bool insertBFS(const View& v, Tag fp, size_t startBucket, uint32_t& chain) {
+        constexpr uint32_t numCandidates = (BS / 2) < 8 ? uint32_t(BS / 2) : 8u;
+        if constexpr (numCandidates > 0) {
+            Word* sb = bucketPtr(v, startBucket);
+            for (uint32_t i = 0; i < numCandidates; ++i) {
+                const size_t slot = (static_cast<size_t>(fp) + static_cast<size_t>(i) * 7) & (BS - 1);
+                const uint32_t w = static_cast<uint32_t>(slot / TPW);
+                const uint32_t lane = static_cast<uint32_t>(slot & (TPW - 1));
+                Word cur = detail::atomicLoadRelaxed(sb + w);
+                const Tag cand = extractTag(cur, lane);
+                if (cand == EMPTY) {
+                    if (tryInsertAtBucket(v, startBucket, fp)) return true;
+                    continue;
+                }
+                const auto alt = altWithNewFp(startBucket, cand, v.numBuckets);
+                if (tryInsertAtBucket(v, alt.bucket, alt.fp)) {
+                    // swap our tag in only while the candidate still sits there
+                    while (extractTag(cur, lane) == cand) {
+                        const Word old = detail::casRelaxed(sb + w, cur, replaceTag(cur, lane, fp));
+                        if (old == cur) {
+                            chain = 1;
+                            return true;
+                        }
+                        cur = old;
+                    }
+                    // :1011-1012 -- undo the copy placed in the alternate bucket
+                    tryRemoveAtBucket(v, alt.bucket, cand);
+                }
+            }
+        }
+        return insertDFS(v, fp, startBucket, chain);
+    }
+
+    /// CuckooFilter::insert from its primary bucket (CuckooFilter.cuh:1029-1080,
+    /// insertKernelSorted :1228-1258).
+    static __device__ __forceinline__ bool insertFrom(const View& v, size_t i1, Tag fp, uint32_t& chain) {
+        chain = 0;
+        if (tryInsertAtBucket(v, i1, fp)) return true;
+        const auto second = altWithNewFp(i1, fp, v.numBuckets);
+        if (tryInsertAtBucket(v, second.bucket, second.fp)) return true;
+        const bool first = (fp & 1) == 0;
+        const Tag evictFp = first ? fp : second.fp;
+        const size_t start = first ? i1 : second.bucket;
+        if (v.evictionPolicy == static_cast<uint32_t>(EvictionPolicy::BFS)) {
+            return insertBFS(v, evictFp, start, chain);
+        }
+        return insertDFS(v, evictFp, start, chain);
+    }
+
+    static __device__ __forceinline__ bool insert(const View& v, const Key& key, uint32_t& chain) {
+        const auto c = Policy::getCandidateBuckets(key, v.numBuckets);
+        return insertFrom(v, c.i1, c.fp, chain);
+    }
+
+    /// CuckooFilter::contains (CuckooFilter.cuh:1088-1098).
+    static __device__ __forceinline__ bool contains(const View& v, const Key& key) {
+        const auto c = Policy::getCandidateBuckets(key, v.numBuckets);
+        if (bucketContains(v, c.i1, c.fp)) return true;
+        Tag fp2 = c.fp;
+        if constexpr (Policy::usesChoiceBit) fp2 = altWithNewFp(c.i1, c.fp, v.numBuckets).fp;
+        if (c.i2 == c.i1 && fp2 == c.fp) return false;
+        return bucketContains(v, c.i2, fp2);
+    }
+
+    /// CuckooFilter::remove (CuckooFilter.cuh:1106-1116).
+    static __device__ __forceinline__ bool remove(const View& v, const Key& key) {
+        const auto c = Policy::getCandidateBuckets(key, v.numBuckets);
+        if (tryRemoveAtBucket(v, c.i1, c.fp)) return true;
+        Tag fp2 = c.fp;
+        if constexpr (Policy::usesChoiceBit) fp2 = altWithNewFp(c.i1, c.fp, v.numBuckets).fp;
+        return tryRemoveAtBucket(v, c.i2, fp2);
+    }
+};
+
+}  // namespace cuckoo

@@ -1,0 +1,305 @@
+// gfx950 kernels of the cuckoo filter and the error-returning launch layer
+// shared by the C++ template API (CuckooFilter.hpp) and the C ABI.
+//
+// Kernel names and argument order follow the reference's free kernel
+// templates (CuckooFilter.cuh:82-128, bodies :1119-1266); the filter argument
+// is a DeviceView passed by value instead of a host `this` pointer.
+//
+// Launch shape: grid-stride loops over keys with the grid capped at the
+// number of blocks the device keeps resident, so a 2^28-key call is a few
+// thousand workgroups rather than a million, and the occupancy counter gets
+// one atomic per workgroup instead of one per 256 keys.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+
+#include "device_ops.hpp"
+#include "radix.hpp"
+
+namespace cuckoo {
+
+inline constexpr int kMaxBlock = 1024;
+
+namespace detail {
+
+/// Adds (or subtracts) a per-thread count to a device counter with one
+/// global atomic per workgroup.  Every thread of the block must call it.
+__device__ __forceinline__ void blockCountAdd(uint32_t local, unsigned long long* target, bool subtract) {
+    __shared__ uint32_t sum;
+    if (threadIdx.x == 0) sum = 0;
+    __syncthreads();
+    // wave64 reduction first, then one LDS atomic per wave
+    uint32_t v = local;
+    const bool fullWaves = (blockDim.x % 64) == 0;
+    if (fullWaves) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (__lane_id() == 0 && v) atomicAdd(&sum, v);
+    } else if (v) {
+        atomicAdd(&sum, v);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && sum) {
+        const unsigned long long d = subtract ? 0ull - static_cast<unsigned long long>(sum)
+                                              : static_cast<unsigned long long>(sum);
+        atomicAdd(target, d);
+    }
+}
+
+}  // namespace detail
+
+/// insertKernel (CuckooFilter.cuh:1119-1141).
+template <typename Config>
+__global__ __launch_bounds__(kMaxBlock) void insertKernel(const typename Config::KeyType* __restrict__ keys,
+                                                          size_t n, DeviceView<Config> v) {
+    using Ops = FilterOps<Config>;
+    uint32_t ok = 0;
+    uint32_t ev = 0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t chain;
+        const bool s = Ops::insert(v, keys[i], chain);
+        ok += s;
+        ev += chain;
+        if (v.chainHist != nullptr && (chain != 0 || !s)) {
+            atomicAdd(&v.chainHist[s ? chain : v.maxEvictions + 1], 1u);
+        }
+    }
+    detail::blockCountAdd(ok, v.occupied, false);
+    if (v.evictions != nullptr) detail::blockCountAdd(ev, v.evictions, false);
+}
+
+/// containsKernel (CuckooFilter.cuh:1143-1155).
+template <typename Config>
+__global__ __launch_bounds__(kMaxBlock) void containsKernel(const typename Config::KeyType* __restrict__ keys,
+                                                            bool* __restrict__ output, size_t n,
+                                                            DeviceView<Config> v) {
+    using Ops = FilterOps<Config>;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        output[i] = Ops::contains(v, keys[i]);
+    }
+}
+
+/// deleteKernel (CuckooFilter.cuh:1157-1183); output may be nullptr.
+template <typename Config>
+__global__ __launch_bounds__(kMaxBlock) void deleteKernel(const typename Config::KeyType* __restrict__ keys,
+                                                          bool* __restrict__ output, size_t n,
+                                                          DeviceView<Config> v) {
+    using Ops = FilterOps<Config>;
+    uint32_t ok = 0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const bool s = Ops::remove(v, keys[i]);
+        ok += s;
+        if (output != nullptr) output[i] = s;
+    }
+    detail::blockCountAdd(ok, v.occupied, true);
+}
+
+/// PackedTagType of a configuration (CuckooFilter.cuh:159).
+template <typename Config>
+using PackedTagOf = std::conditional_t<Config::bitsPerTag <= 8, uint32_t, uint64_t>;
+
+/// computePackedTagsKernel (CuckooFilter.cuh:1185-1207): (i1 << bitsPerTag) | fp
+/// in PackedTagType (so i1 is truncated exactly as the reference truncates it).
+template <typename Config>
+__global__ __launch_bounds__(kMaxBlock) void computePackedTagsKernel(
+    const typename Config::KeyType* __restrict__ keys, PackedTagOf<Config>* __restrict__ packedTags, size_t n,
+    size_t numBuckets) {
+    using P = PackedTagOf<Config>;
+    using Policy = typename Config::AltBucketPolicy;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const auto c = Policy::getCandidateBuckets(keys[i], numBuckets);
+        packedTags[i] = (static_cast<P>(c.i1) << Config::bitsPerTag) | static_cast<P>(c.fp);
+    }
+}
+
+/// insertKernelSorted (CuckooFilter.cuh:1209-1266); `keys` is unused, as there.
+template <typename Config>
+__global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename Config::KeyType* /*keys*/,
+                                                                const PackedTagOf<Config>* __restrict__ packedTags,
+                                                                size_t n, DeviceView<Config> v) {
+    using Ops = FilterOps<Config>;
+    using Tag = typename Config::TagType;
+    using P = PackedTagOf<Config>;
+    constexpr P fpMask = static_cast<P>((1ULL << Config::bitsPerTag) - 1);
+    uint32_t ok = 0;
+    uint32_t ev = 0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const P packed = packedTags[i];
+        uint32_t chain;
+        const bool s = Ops::insertFrom(v, static_cast<size_t>(packed >> Config::bitsPerTag),
+                                       static_cast<Tag>(packed & fpMask), chain);
+        ok += s;
+        ev += chain;
+        if (v.chainHist != nullptr && (chain != 0 || !s)) {
+            atomicAdd(&v.chainHist[s ? chain : v.maxEvictions + 1], 1u);
+        }
+    }
+    detail::blockCountAdd(ok, v.occupied, false);
+    if (v.evictions != nullptr) detail::blockCountAdd(ev, v.evictions, false);
+}
+
+/// Per-key (i1, i2, fp) exactly as getCandidateBuckets computes them.
+template <typename Config>
+__global__ __launch_bounds__(kMaxBlock) void candidatesKernel(const typename Config::KeyType* __restrict__ keys,
+                                                              size_t n, size_t numBuckets,
+                                                              uint64_t* __restrict__ i1, uint64_t* __restrict__ i2,
+                                                              uint32_t* __restrict__ fp) {
+    using Policy = typename Config::AltBucketPolicy;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const auto c = Policy::getCandidateBuckets(keys[i], numBuckets);
+        i1[i] = c.i1;
+        i2[i] = c.i2;
+        fp[i] = c.fp;
+    }
+}
+
+/// Device memory the sorted insert needs, grown on demand and kept between calls.
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+
+    hipError_t reserve(size_t want) {
+        if (want <= bytes) return hipSuccess;
+        if (ptr != nullptr) {
+            const hipError_t e = hipFree(ptr);
+            ptr = nullptr;
+            bytes = 0;
+            if (e != hipSuccess) return e;
+        }
+        const hipError_t e = hipMalloc(&ptr, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+
+    void release() {
+        if (ptr != nullptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+namespace detail {
+
+/// Grid for a grid-stride kernel: enough blocks for n keys, capped at
+/// `wavesPerSimd` resident waves on every SIMD of the device.
+template <typename Kernel>
+inline uint32_t gridFor(Kernel kernel, size_t n, uint32_t block) {
+    const size_t need = (n + block - 1) / block;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    static thread_local int cachedDev = -1;
+    static thread_local int cus = 0;
+    if (cachedDev != dev) {
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        cachedDev = dev;
+    }
+    int perCu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, static_cast<int>(block), 0) != hipSuccess ||
+        perCu <= 0) {
+        perCu = 1;
+    }
+    const size_t cap = static_cast<size_t>(perCu) * static_cast<size_t>(std::max(cus, 1));
+    return static_cast<uint32_t>(std::max<size_t>(1, std::min(need, cap)));
+}
+
+inline int bitsFor(size_t nb) {  // ceil(log2(nb))
+    int b = 0;
+    while ((size_t(1) << b) < nb) ++b;
+    return b;
+}
+
+}  // namespace detail
+
+/// Error-returning launches for one configuration.  Every call is
+/// asynchronous on `stream`; callers synchronise.
+template <typename Config>
+struct Engine {
+    using Key = typename Config::KeyType;
+    using View = DeviceView<Config>;
+    using Packed = PackedTagOf<Config>;
+
+    static hipError_t insert(const View& v, const Key* keys, size_t n, uint32_t block, hipStream_t s) {
+        if (n == 0) return hipSuccess;
+        auto k = insertKernel<Config>;
+        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, n, v);
+        return hipGetLastError();
+    }
+
+    static hipError_t contains(const View& v, const Key* keys, size_t n, bool* out, uint32_t block,
+                               hipStream_t s) {
+        if (n == 0) return hipSuccess;
+        auto k = containsKernel<Config>;
+        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, out, n, v);
+        return hipGetLastError();
+    }
+
+    static hipError_t remove(const View& v, const Key* keys, size_t n, bool* out, uint32_t block,
+                             hipStream_t s) {
+        if (n == 0) return hipSuccess;
+        auto k = deleteKernel<Config>;
+        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, out, n, v);
+        return hipGetLastError();
+    }
+
+    static hipError_t packedTags(const Key* keys, size_t n, size_t nb, Packed* out, uint32_t block,
+                                 hipStream_t s) {
+        if (n == 0) return hipSuccess;
+        auto k = computePackedTagsKernel<Config>;
+        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, out, n, nb);
+        return hipGetLastError();
+    }
+
+    static hipError_t candidates(const Key* keys, size_t n, size_t nb, uint64_t* i1, uint64_t* i2,
+                                 uint32_t* fp, uint32_t block, hipStream_t s) {
+        if (n == 0) return hipSuccess;
+        auto k = candidatesKernel<Config>;
+        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, n, nb, i1, i2, fp);
+        return hipGetLastError();
+    }
+
+    /// Bytes of workspace insertSorted needs for a chunk of n keys.
+    static size_t sortedWorkspaceBytes(size_t n) {
+        const size_t vals = ((n * sizeof(Packed) + 255) / 256) * 256;
+        return 2 * vals + radix::scratchBytes(n);
+    }
+
+    /// insertManySorted (CuckooFilter.cuh:475-508): packed tags, ordered by
+    /// primary bucket, then the sorted insert kernel.  Chunks of at most 2^31
+    /// keys (radix counters are 32-bit).
+    static hipError_t insertSorted(const View& v, const Key* keys, size_t n, uint32_t block, hipStream_t s,
+                                   Workspace& ws) {
+        constexpr size_t kChunk = size_t(1) << 31;
+        for (size_t off = 0; off < n; off += kChunk) {
+            const size_t m = std::min(kChunk, n - off);
+            hipError_t e = ws.reserve(sortedWorkspaceBytes(m));
+            if (e != hipSuccess) return e;
+            const size_t vals = ((m * sizeof(Packed) + 255) / 256) * 256;
+            Packed* a = static_cast<Packed*>(ws.ptr);
+            Packed* b = reinterpret_cast<Packed*>(static_cast<char*>(ws.ptr) + vals);
+            void* scratch = static_cast<char*>(ws.ptr) + 2 * vals;
+            if ((e = packedTags(keys + off, m, v.numBuckets, a, block, s)) != hipSuccess) return e;
+            int hi = Config::bitsPerTag + detail::bitsFor(v.numBuckets);
+            hi = std::min<int>(hi, static_cast<int>(sizeof(Packed) * 8));
+            Packed* sorted = a;
+            if ((e = radix::sortBits<Packed>(a, b, m, Config::bitsPerTag, hi, scratch, s, &sorted)) != hipSuccess) {
+                return e;
+            }
+            auto k = insertKernelSorted<Config>;
+            k<<<detail::gridFor(k, m, block), block, 0, s>>>(nullptr, sorted, m, v);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+};
+
+}  // namespace cuckoo

@@ -1,0 +1,333 @@
+"""Python host layer over libcuckoo_amd.so (the C ABI in include/cuckoo_filter.h).
+
+Mirrors the reference's CuckooFilter<Config> interface
+(tiltedtoast/cuckoo-filter include/CuckooFilter.cuh:139-785): the same
+method names, argument meaning and return values (insert / delete return the
+total occupied slots after the call).  Device buffers are torch tensors on a
+ROCm device; torch is only the allocator and stream source here -- every
+filter operation runs in the HIP kernels of libcuckoo_amd.so.  If the library
+is missing this module raises at import time: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcuckoo_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cuckoo_filter.h")
+
+CF_OK, CF_ERR_INVALID, CF_ERR_UNSUPPORTED, CF_ERR_HIP, CF_ERR_ALLOC = 0, -1, -2, -3, -4
+POLICIES = {"xor": 0, "addsub": 1, "offset": 2}
+EVICTIONS = {"bfs": 0, "dfs": 1}
+
+
+class CuckooError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cuckoo filter error {code}: {msg}")
+        self.code = code
+
+
+class cf_config(C.Structure):
+    _fields_ = [
+        ("key_bytes", C.c_uint32),
+        ("bits_per_tag", C.c_uint32),
+        ("max_evictions", C.c_uint32),
+        ("block_size", C.c_uint32),
+        ("bucket_size", C.c_uint32),
+        ("policy", C.c_uint32),
+        ("eviction", C.c_uint32),
+        ("word_bytes", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run python cuckoo-filter_amd/build.py "
+                          "(or __graft_entry__.build()) first")
+    L = C.CDLL(LIB_PATH)
+    P, sz, i32 = C.c_void_p, C.c_size_t, C.c_int
+    cfgp = C.POINTER(cf_config)
+    u64p, u32p = C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)
+    sig = {
+        "cf_last_error": (C.c_char_p, []),
+        "cf_config_init": (i32, [cfgp, C.c_uint32, C.c_uint32]),
+        "cf_config_supported": (i32, [cfgp]),
+        "cf_calculate_num_buckets": (sz, [cfgp, sz]),
+        "cf_candidate_buckets_host": (i32, [cfgp, sz, P, sz, P, P, P]),
+        "cf_create": (i32, [cfgp, sz, C.POINTER(P)]),
+        "cf_destroy": (i32, [P]),
+        "cf_insert_many": (i32, [P, P, sz, P, C.POINTER(sz)]),
+        "cf_insert_many_sorted": (i32, [P, P, sz, P, C.POINTER(sz)]),
+        "cf_contains_many": (i32, [P, P, sz, P, P]),
+        "cf_delete_many": (i32, [P, P, sz, P, P, C.POINTER(sz)]),
+        "cf_clear": (i32, [P]),
+        "cf_occupied_slots": (i32, [P, C.POINTER(sz)]),
+        "cf_load_factor": (i32, [P, C.POINTER(C.c_float)]),
+        "cf_capacity": (sz, [P]),
+        "cf_num_buckets": (sz, [P]),
+        "cf_size_in_bytes": (sz, [P]),
+        "cf_count_occupied_slots": (i32, [P, C.POINTER(sz)]),
+        "cf_eviction_count": (i32, [P, C.POINTER(sz)]),
+        "cf_reset_eviction_count": (i32, [P]),
+        "cf_set_eviction_histogram": (i32, [P, i32]),
+        "cf_eviction_histogram": (i32, [P, P, sz]),
+        "cf_export_table": (i32, [P, P, sz]),
+        "cf_import_table": (i32, [P, P, sz, sz]),
+        "cf_compute_candidates": (i32, [P, P, sz, P, P, P, P]),
+        "cf_compute_packed_tags": (i32, [P, P, sz, P, P]),
+    }
+    del u64p, u32p
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def header_symbols() -> list[str]:
+    """Every function the public header declares."""
+    text = open(HEADER_PATH).read()
+    decl = r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z_0-9]*\s*\**\s+(cf_[a-z_0-9]+)\s*\("
+    return sorted(set(re.findall(decl, text, flags=re.M)))
+
+
+def _check(rc: int):
+    if rc != CF_OK:
+        raise CuckooError(rc, lib.cf_last_error().decode(errors="replace"))
+    return rc
+
+
+def make_config(key_bytes=8, bits_per_tag=16, max_evictions=500, block_size=256, bucket_size=16,
+                policy="xor", eviction="dfs", word_bytes=8) -> cf_config:
+    """Run-time CuckooConfig; defaults are the reference's (CuckooFilter.cuh:44-52)."""
+    c = cf_config()
+    _check(lib.cf_config_init(C.byref(c), key_bytes, bits_per_tag))
+    c.max_evictions = max_evictions
+    c.block_size = block_size
+    c.bucket_size = bucket_size
+    c.policy = POLICIES[policy] if isinstance(policy, str) else int(policy)
+    c.eviction = EVICTIONS[eviction] if isinstance(eviction, str) else int(eviction)
+    c.word_bytes = word_bytes
+    return c
+
+
+def config_supported(cfg: cf_config) -> bool:
+    r = lib.cf_config_supported(C.byref(cfg))
+    if r < 0:
+        raise CuckooError(r, lib.cf_last_error().decode(errors="replace"))
+    return r == 1
+
+
+def calculate_num_buckets(cfg: cf_config, capacity: int) -> int:
+    return int(lib.cf_calculate_num_buckets(C.byref(cfg), capacity))
+
+
+def candidate_buckets_host(cfg: cf_config, num_buckets: int, keys: np.ndarray):
+    """Host getCandidateBuckets (the reference's __host__ static)."""
+    dt = np.uint64 if cfg.key_bytes == 8 else np.uint32
+    k = np.ascontiguousarray(keys, dtype=dt)
+    i1 = np.empty(k.size, np.uint64)
+    i2 = np.empty(k.size, np.uint64)
+    fp = np.empty(k.size, np.uint32)
+    _check(lib.cf_candidate_buckets_host(C.byref(cfg), num_buckets, k.ctypes.data, k.size,
+                                         i1.ctypes.data, i2.ctypes.data, fp.ctypes.data))
+    return i1, i2, fp
+
+
+def _torch():
+    import torch  # deferred: only device operations need it
+
+    return torch
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(int(stream.cuda_stream))
+
+
+class CuckooFilter:
+    """CuckooFilter<Config> over device tensors (CuckooFilter.cuh:139-785)."""
+
+    def __init__(self, capacity: int, config: cf_config | None = None, **cfg_kwargs):
+        self.config = config if config is not None else make_config(**cfg_kwargs)
+        h = C.c_void_p()
+        _check(lib.cf_create(C.byref(self.config), capacity, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.cf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------------
+    def _keys(self, keys):
+        torch = _torch()
+        if not isinstance(keys, torch.Tensor) or not keys.is_cuda:
+            raise TypeError("keys must be a device tensor")
+        if keys.element_size() != self.config.key_bytes or keys.dtype.is_floating_point:
+            raise TypeError(f"keys must be {8 * self.config.key_bytes}-bit integers, got {keys.dtype}")
+        if not keys.is_contiguous():
+            raise ValueError("keys must be contiguous")
+        return keys
+
+    def _out(self, out, n):
+        torch = _torch()
+        if out is None:
+            return None
+        if out.dtype not in (torch.bool, torch.uint8) or out.numel() < n or not out.is_cuda:
+            raise TypeError("output must be a device bool/uint8 tensor with one entry per key")
+        return out
+
+    def insertMany(self, keys, stream=None) -> int:
+        k = self._keys(keys)
+        occ = C.c_size_t()
+        _check(lib.cf_insert_many(self._h, C.c_void_p(k.data_ptr()), k.numel(), _stream_ptr(stream),
+                                  C.byref(occ)))
+        return occ.value
+
+    def insertManySorted(self, keys, stream=None) -> int:
+        k = self._keys(keys)
+        occ = C.c_size_t()
+        _check(lib.cf_insert_many_sorted(self._h, C.c_void_p(k.data_ptr()), k.numel(), _stream_ptr(stream),
+                                         C.byref(occ)))
+        return occ.value
+
+    def containsMany(self, keys, out=None, stream=None):
+        torch = _torch()
+        k = self._keys(keys)
+        if out is None:
+            out = torch.empty(k.numel(), dtype=torch.bool, device=k.device)
+        self._out(out, k.numel())
+        _check(lib.cf_contains_many(self._h, C.c_void_p(k.data_ptr()), k.numel(), C.c_void_p(out.data_ptr()),
+                                    _stream_ptr(stream)))
+        return out
+
+    def deleteMany(self, keys, out=None, stream=None) -> int:
+        k = self._keys(keys)
+        self._out(out, k.numel())
+        occ = C.c_size_t()
+        optr = C.c_void_p(out.data_ptr()) if out is not None else None
+        _check(lib.cf_delete_many(self._h, C.c_void_p(k.data_ptr()), k.numel(), optr, _stream_ptr(stream),
+                                  C.byref(occ)))
+        return occ.value
+
+    def clear(self):
+        _check(lib.cf_clear(self._h))
+
+    def loadFactor(self) -> float:
+        v = C.c_float()
+        _check(lib.cf_load_factor(self._h, C.byref(v)))
+        return v.value
+
+    def occupiedSlots(self) -> int:
+        v = C.c_size_t()
+        _check(lib.cf_occupied_slots(self._h, C.byref(v)))
+        return v.value
+
+    def capacity(self) -> int:
+        return int(lib.cf_capacity(self._h))
+
+    def getNumBuckets(self) -> int:
+        return int(lib.cf_num_buckets(self._h))
+
+    @property
+    def numBuckets(self) -> int:
+        return self.getNumBuckets()
+
+    def sizeInBytes(self) -> int:
+        return int(lib.cf_size_in_bytes(self._h))
+
+    def countOccupiedSlots(self) -> int:
+        v = C.c_size_t()
+        _check(lib.cf_count_occupied_slots(self._h, C.byref(v)))
+        return v.value
+
+    def evictionCount(self) -> int:
+        v = C.c_size_t()
+        _check(lib.cf_eviction_count(self._h, C.byref(v)))
+        return v.value
+
+    def resetEvictionCount(self):
+        _check(lib.cf_reset_eviction_count(self._h))
+
+    # ------------------------------------------------------------------
+    # additions: histogram, table images, device hashing
+    def setEvictionHistogram(self, enable: bool = True):
+        _check(lib.cf_set_eviction_histogram(self._h, 1 if enable else 0))
+
+    def evictionHistogram(self) -> np.ndarray:
+        bins = int(self.config.max_evictions) + 2
+        out = np.zeros(bins, np.uint64)
+        _check(lib.cf_eviction_histogram(self._h, out.ctypes.data, bins))
+        return out
+
+    def exportTable(self) -> np.ndarray:
+        img = np.empty(self.sizeInBytes(), np.uint8)
+        _check(lib.cf_export_table(self._h, img.ctypes.data, img.size))
+        return img
+
+    def importTable(self, image: np.ndarray, occupied: int):
+        img = np.ascontiguousarray(image, dtype=np.uint8)
+        _check(lib.cf_import_table(self._h, img.ctypes.data, img.size, occupied))
+
+    def computeCandidates(self, keys, stream=None):
+        torch = _torch()
+        k = self._keys(keys)
+        n = k.numel()
+        i1 = torch.empty(n, dtype=torch.int64, device=k.device)
+        i2 = torch.empty(n, dtype=torch.int64, device=k.device)
+        fp = torch.empty(n, dtype=torch.int32, device=k.device)
+        _check(lib.cf_compute_candidates(self._h, C.c_void_p(k.data_ptr()), n, C.c_void_p(i1.data_ptr()),
+                                         C.c_void_p(i2.data_ptr()), C.c_void_p(fp.data_ptr()),
+                                         _stream_ptr(stream)))
+        return i1, i2, fp
+
+    def computePackedTags(self, keys, stream=None):
+        torch = _torch()
+        k = self._keys(keys)
+        dt = torch.int32 if self.config.bits_per_tag <= 8 else torch.int64
+        out = torch.empty(k.numel(), dtype=dt, device=k.device)
+        _check(lib.cf_compute_packed_tags(self._h, C.c_void_p(k.data_ptr()), k.numel(),
+                                          C.c_void_p(out.data_ptr()), _stream_ptr(stream)))
+        return out
+
+
+def keys_to_device(keys: np.ndarray, device="cuda"):
+    """Host uint64/uint32 array -> device tensor holding the same bit patterns."""
+    torch = _torch()
+    if keys.dtype == np.uint64:
+        t = torch.from_numpy(keys.view(np.int64))
+    elif keys.dtype == np.uint32:
+        t = torch.from_numpy(keys.view(np.int32))
+    else:
+        raise TypeError(keys.dtype)
+    return t.to(device)
+
+
+def to_numpy_u(t, dtype) -> np.ndarray:
+    """Device integer tensor -> host numpy array reinterpreted as unsigned dtype."""
+    return t.cpu().numpy().view(dtype)

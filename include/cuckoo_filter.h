@@ -1,0 +1,149 @@
+/*
+ * cuckoo_filter.h -- C ABI of the MI355X-native cuckoo filter (libcuckoo_amd.so).
+ *
+ * The reference (tiltedtoast/cuckoo-filter) is a header-only C++ template
+ * library with no C ABI; its boundary is CuckooFilter<Config>
+ * (include/CuckooFilter.cuh:139-1117).  This header exposes that same
+ * boundary for fixed instantiations selected at run time by cf_config, so
+ * that non-C++ hosts (ctypes, cgo, JNI, N-API) can bind it.  C++ callers can
+ * instead include cuckoo-filter_amd/csrc/cuckoo/CuckooFilter.hpp, the
+ * template drop-in.
+ *
+ * Conventions
+ *   - All key / output pointers marked d_ are device pointers on the device
+ *     that was current when the filter was created; h_ pointers are host.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Every operation
+ *     is synchronous on that stream on return, like the reference's methods.
+ *   - Functions return CF_OK (0) or a negative CF_ERR_* code; the message of
+ *     the last failure on the calling thread is cf_last_error().  (The C++
+ *     template API keeps the reference's print-and-exit convention instead.)
+ *   - Insert / delete report the TOTAL number of occupied slots after the
+ *     call through *occupied (CuckooFilter.cuh:463,547), not a per-call count.
+ */
+#ifndef CUCKOO_FILTER_AMD_H
+#define CUCKOO_FILTER_AMD_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CF_OK 0
+#define CF_ERR_INVALID (-1)     /* bad argument or configuration */
+#define CF_ERR_UNSUPPORTED (-2) /* configuration valid but not built into this library */
+#define CF_ERR_HIP (-3)         /* HIP runtime / launch error */
+#define CF_ERR_ALLOC (-4)       /* device or host allocation failed */
+
+enum cf_policy { CF_POLICY_XOR = 0, CF_POLICY_ADDSUB = 1, CF_POLICY_OFFSET = 2 };
+enum cf_eviction { CF_EVICT_BFS = 0, CF_EVICT_DFS = 1 };
+
+/* Run-time form of CuckooConfig<T, bitsPerTag, maxEvictions, blockSize,
+ * bucketSize, AltBucketPolicy, evictionPolicy, WordType>
+ * (CuckooFilter.cuh:44-74). */
+typedef struct cf_config {
+    uint32_t key_bytes;     /* sizeof(T): 4 (uint32_t keys) or 8 (uint64_t keys) */
+    uint32_t bits_per_tag;  /* 8, 16 or 32 */
+    uint32_t max_evictions; /* maxEvictions (default 500) */
+    uint32_t block_size;    /* blockSize: threads per workgroup, 1..1024 (default 256) */
+    uint32_t bucket_size;   /* bucketSize: slots per bucket, power of two (default 16) */
+    uint32_t policy;        /* enum cf_policy (default XOR) */
+    uint32_t eviction;      /* enum cf_eviction (default DFS, CuckooFilter.cuh:51) */
+    uint32_t word_bytes;    /* sizeof(WordType): 4 or 8 (default 8) */
+} cf_config;
+
+typedef struct cf_filter cf_filter;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* cf_last_error(void);
+
+/* Fills cfg with CuckooConfig's defaults for the given key width and tag bits
+ * (CuckooFilter.cuh:44-52). */
+int cf_config_init(cf_config* cfg, uint32_t key_bytes, uint32_t bits_per_tag);
+
+/* 1 if this library holds kernels for cfg, 0 if not, CF_ERR_INVALID if cfg
+ * violates the reference's static_asserts (CuckooFilter.cuh:67-71,147-152,
+ * 226-236). */
+int cf_config_supported(const cf_config* cfg);
+
+/* calculateNumBuckets (CuckooFilter.cuh:406-408 -> bucket_policies.cuh:80-83,
+ * 172-181, 347-350). */
+size_t cf_calculate_num_buckets(const cf_config* cfg, size_t capacity);
+
+/* Host-side getCandidateBuckets for n host keys (bucket_policies.cuh:45-61,
+ * 126-143, 263-285; a __host__ __device__ static in the reference). */
+int cf_candidate_buckets_host(const cf_config* cfg, size_t num_buckets, const void* h_keys, size_t n,
+                              uint64_t* h_i1, uint64_t* h_i2, uint32_t* h_fp);
+
+/* explicit CuckooFilter(size_t capacity)  (CuckooFilter.cuh:420-428) */
+int cf_create(const cf_config* cfg, size_t capacity, cf_filter** out);
+
+/* ~CuckooFilter()  (CuckooFilter.cuh:435-447) */
+int cf_destroy(cf_filter* f);
+
+/* size_t insertMany(const T* d_keys, size_t n, cudaStream_t)  (CuckooFilter.cuh:457-464) */
+int cf_insert_many(cf_filter* f, const void* d_keys, size_t n, void* stream, size_t* occupied);
+
+/* size_t insertManySorted(const T* d_keys, size_t n, cudaStream_t)  (CuckooFilter.cuh:475-508) */
+int cf_insert_many_sorted(cf_filter* f, const void* d_keys, size_t n, void* stream, size_t* occupied);
+
+/* void containsMany(const T* d_keys, size_t n, bool* d_output, cudaStream_t)  (CuckooFilter.cuh:518-523) */
+int cf_contains_many(cf_filter* f, const void* d_keys, size_t n, bool* d_output, void* stream);
+
+/* size_t deleteMany(const T* d_keys, size_t n, bool* d_output = nullptr, cudaStream_t)
+ * (CuckooFilter.cuh:536-548); d_output may be NULL. */
+int cf_delete_many(cf_filter* f, const void* d_keys, size_t n, bool* d_output, void* stream,
+                   size_t* occupied);
+
+/* void clear()  (CuckooFilter.cuh:675-683) */
+int cf_clear(cf_filter* f);
+
+/* size_t occupiedSlots()  (CuckooFilter.cuh:699-704) */
+int cf_occupied_slots(cf_filter* f, size_t* occupied);
+
+/* float loadFactor()  (CuckooFilter.cuh:688-690) */
+int cf_load_factor(cf_filter* f, float* load_factor);
+
+/* size_t capacity() / getNumBuckets() / sizeInBytes()  (CuckooFilter.cuh:732-750) */
+size_t cf_capacity(const cf_filter* f);
+size_t cf_num_buckets(const cf_filter* f);
+size_t cf_size_in_bytes(const cf_filter* f);
+
+/* size_t countOccupiedSlots()  (CuckooFilter.cuh:759-785) */
+int cf_count_occupied_slots(cf_filter* f, size_t* count);
+
+/* evictionCount() / resetEvictionCount()  (CuckooFilter.cuh:706-726; always
+ * counted here, as with CUCKOO_FILTER_COUNT_EVICTIONS). */
+int cf_eviction_count(cf_filter* f, size_t* count);
+int cf_reset_eviction_count(cf_filter* f);
+
+/* Eviction-chain histogram of subsequent inserts (no reference counterpart;
+ * extends the CUCKOO_FILTER_COUNT_EVICTIONS counter, eviction_benchmark.cu).
+ * bins = max_evictions + 2: bin c (1 <= c <= max_evictions) counts successful
+ * inserts that made c swaps, bin max_evictions+1 counts failed inserts; bin 0
+ * is left 0 (callers derive it as keys - sum).  Enabling resets the bins. */
+int cf_set_eviction_histogram(cf_filter* f, int enable);
+int cf_eviction_histogram(cf_filter* f, uint64_t* h_bins, size_t nbins);
+
+/* Raw table image (numBuckets * sizeof(Bucket) bytes, little-endian words;
+ * CuckooFilter.cuh:237,350,748-750) to / from host memory.  Import also sets
+ * the occupancy counter. */
+int cf_export_table(cf_filter* f, void* h_dst, size_t bytes);
+int cf_import_table(cf_filter* f, const void* h_src, size_t bytes, size_t occupied);
+
+/* getCandidateBuckets on the device for n device keys (the device half of the
+ * reference's __host__ __device__ static, CuckooFilter.cuh:366-369). */
+int cf_compute_candidates(cf_filter* f, const void* d_keys, size_t n, uint64_t* d_i1, uint64_t* d_i2,
+                          uint32_t* d_fp, void* stream);
+
+/* computePackedTagsKernel (CuckooFilter.cuh:1185-1207): d_packed receives n
+ * PackedTagType values (uint32_t when bits_per_tag <= 8, else uint64_t). */
+int cf_compute_packed_tags(cf_filter* f, const void* d_keys, size_t n, void* d_packed, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUCKOO_FILTER_AMD_H */

@@ -1,0 +1,288 @@
+"""Parity of the HIP kernels (through the C ABI) with the oracle.
+
+Bit-exact where the reference is deterministic:
+  * device getCandidateBuckets / computePackedTags vs the golden vectors;
+  * containsMany on a given table image (oracle-built and hand-built) vs the
+    oracle's lookups;
+  * one-key-per-call insertMany / insertManySorted / deleteMany -- each call
+    is a whole kernel launch, so the calls are strictly ordered and the table
+    image after every call must equal the sequential oracle's, eviction paths
+    and exhausted chains included.
+Statistical / invariant where the reference itself races (concurrent batch
+inserts, CuckooFilter.cuh:1119-1141): occupancy == non-empty slots, stored
+tags == the keys' fingerprints, every stored tag sits in one of its key's
+buckets, no false negatives, failure count and FPR within binomial bounds of
+the oracle at the same load.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_cases
+
+pytestmark = pytest.mark.gpu
+
+TAG_DT = {8: np.uint8, 16: np.uint16, 32: np.uint32}
+
+
+def _dev_keys(cf, keys):
+    return cf.keys_to_device(np.ascontiguousarray(keys))
+
+
+def _oracle_cfg(O, c):
+    d = c.as_dict()
+    return O.make_config(**d)
+
+
+def _tags(image, c):
+    return np.frombuffer(image.tobytes(), dtype=TAG_DT[c.bits_per_tag]).reshape(-1, c.bucket_size)
+
+
+# ---------------------------------------------------------------------------
+# hashing
+# ---------------------------------------------------------------------------
+
+def test_device_candidates_and_packed_tags_match_golden(cf, golden):
+    for name in golden_cases(golden):
+        pol, kb, bits, nb = (int(x) for x in golden[name + "/meta"])
+        c = cf.make_config(key_bytes=kb, bits_per_tag=bits, policy=pol, bucket_size=8, word_bytes=8)
+        f = cf.CuckooFilter(nb * 8, config=c)
+        assert f.getNumBuckets() == nb, name
+        k = _dev_keys(cf, golden[name + "/keys"])
+        i1, i2, fp = f.computeCandidates(k)
+        np.testing.assert_array_equal(cf.to_numpy_u(i1, np.uint64), golden[name + "/i1"], err_msg=name)
+        np.testing.assert_array_equal(cf.to_numpy_u(i2, np.uint64), golden[name + "/i2"], err_msg=name)
+        np.testing.assert_array_equal(cf.to_numpy_u(fp, np.uint32), golden[name + "/fp"], err_msg=name)
+        packed = f.computePackedTags(k)
+        pdt = np.uint32 if bits <= 8 else np.uint64
+        np.testing.assert_array_equal(cf.to_numpy_u(packed, pdt).astype(np.uint64), golden[name + "/packed"],
+                                      err_msg=name)
+        f.close()
+
+
+# ---------------------------------------------------------------------------
+# lookup on a given image: bit-exact
+# ---------------------------------------------------------------------------
+
+LOOKUP_CFGS = [
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8),
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=4),
+    dict(bits_per_tag=8, bucket_size=32, word_bytes=8),
+    dict(bits_per_tag=8, bucket_size=4, word_bytes=4),
+    dict(bits_per_tag=32, bucket_size=32, word_bytes=8),   # 128-byte buckets
+    dict(bits_per_tag=32, bucket_size=4, word_bytes=4),
+    dict(bits_per_tag=16, bucket_size=4, word_bytes=4),    # the reference's undefined 128-bit case
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, policy="addsub"),
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, policy="offset"),
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, key_bytes=4),
+]
+
+
+@pytest.mark.parametrize("kw", LOOKUP_CFGS, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_contains_bit_exact_on_oracle_image(cf, oracle_mod, kw):
+    O = oracle_mod
+    c = cf.make_config(eviction="bfs", **kw)
+    oc = _oracle_cfg(O, c)
+    cap = 1 << 16
+    of = O.Filter(oc, cap)
+    f = cf.CuckooFilter(cap, config=c)
+    assert f.getNumBuckets() == of.num_buckets and f.sizeInBytes() == of.size_bytes
+    n = int(of.capacity() * 0.9)
+    keys = O.gen_keys(11, n + 60000, c.key_bytes)
+    occ, _ = of.insert_many(keys[:n])
+    f.importTable(of.table_bytes(), occ)
+    assert f.occupiedSlots() == occ
+    q = np.concatenate([keys[: n // 2], keys[n:]])
+    got = f.containsMany(_dev_keys(cf, q)).cpu().numpy()
+    np.testing.assert_array_equal(got, of.contains_many(q))
+    np.testing.assert_array_equal(f.exportTable(), of.table_bytes())
+
+
+def test_contains_bit_exact_on_hand_built_images(cf, oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(3)
+    for kw in LOOKUP_CFGS:
+        c = cf.make_config(**kw)
+        oc = _oracle_cfg(O, c)
+        of = O.Filter(oc, 512 * c.bucket_size)
+        nb = of.num_buckets
+        keys = O.gen_keys(21, 20000, c.key_bytes)
+        i1, i2, fp = O.candidates(oc, nb, keys)
+        tmax = (1 << c.bits_per_tag) - 1
+        tags = rng.integers(0, tmax + 1, size=(nb, c.bucket_size)).astype(np.uint64)
+        tags[rng.random(tags.shape) < 0.3] = 0
+        tags[rng.random(tags.shape) < 0.05] = 1
+        tags[::5] = tags[::5, :1]  # duplicated tags / full buckets
+        for j in rng.choice(keys.size, 2000, replace=False):
+            b, t = (i1[j], fp[j]) if rng.random() < 0.5 else (i2[j], O.alt_bucket(oc, nb, int(i1[j]), int(fp[j]))[1])
+            tags[b, rng.integers(0, c.bucket_size)] = t
+        image = tags.astype(TAG_DT[c.bits_per_tag]).view(np.uint8).reshape(-1)
+        of.load_table(image, int((tags != 0).sum()))
+        f = cf.CuckooFilter(512 * c.bucket_size, config=c)
+        f.importTable(image, int((tags != 0).sum()))
+        got = f.containsMany(_dev_keys(cf, keys)).cpu().numpy()
+        want = of.contains_many(keys)
+        np.testing.assert_array_equal(got, want, err_msg=str(kw))
+        assert want.any() and (~want).any()
+        f.close()
+
+
+# ---------------------------------------------------------------------------
+# strictly ordered single-key calls: bit-exact table images
+# ---------------------------------------------------------------------------
+
+SEQ_CFGS = [
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, eviction="bfs", max_evictions=40),
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, eviction="dfs", max_evictions=40),
+    dict(bits_per_tag=16, bucket_size=4, word_bytes=8, eviction="bfs", max_evictions=25),
+    dict(bits_per_tag=8, bucket_size=16, word_bytes=4, eviction="dfs", max_evictions=30),
+    dict(bits_per_tag=32, bucket_size=8, word_bytes=8, eviction="bfs", max_evictions=30),
+    dict(bits_per_tag=32, bucket_size=32, word_bytes=4, eviction="bfs", max_evictions=30),
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, eviction="bfs", policy="addsub", max_evictions=30),
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, eviction="dfs", policy="offset", max_evictions=30),
+    dict(bits_per_tag=8, bucket_size=8, word_bytes=8, eviction="bfs", policy="offset", max_evictions=30),
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, eviction="dfs", key_bytes=4, max_evictions=30),
+]
+
+
+@pytest.mark.parametrize("kw", SEQ_CFGS, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_single_key_calls_bit_exact(cf, oracle_mod, kw):
+    O = oracle_mod
+    c = cf.make_config(**kw)
+    oc = _oracle_cfg(O, c)
+    cap = 64 * c.bucket_size
+    of = O.Filter(oc, cap)
+    f = cf.CuckooFilter(cap, config=c)
+    n = int(of.capacity() * 1.06)
+    keys = O.gen_keys(31, n, c.key_bytes)
+    dk = _dev_keys(cf, keys)
+    fails = 0
+    for i in range(n):
+        occ_gpu = f.insertMany(dk[i:i + 1])
+        occ_cpu, ok = of.insert_many(keys[i:i + 1])
+        fails += int(not ok[0])
+        assert occ_gpu == occ_cpu, (i, occ_gpu, occ_cpu)
+    np.testing.assert_array_equal(f.exportTable(), of.table_bytes())
+    assert fails > 0  # exhausted chains were exercised
+    assert f.evictionCount() == of.evictions()
+    # ordered single-key deletes of every other key
+    out = __import__("torch").zeros(1, dtype=__import__("torch").bool, device=dk.device)
+    for i in range(0, n, 2):
+        occ_gpu = f.deleteMany(dk[i:i + 1], out=out)
+        occ_cpu, ok = of.delete_many(keys[i:i + 1])
+        assert occ_gpu == occ_cpu and bool(out.item()) == bool(ok[0]), i
+    np.testing.assert_array_equal(f.exportTable(), of.table_bytes())
+
+
+def test_single_key_sorted_calls_bit_exact(cf, oracle_mod):
+    O = oracle_mod
+    c = cf.make_config(bits_per_tag=16, eviction="bfs", max_evictions=40)
+    oc = _oracle_cfg(O, c)
+    of = O.Filter(oc, 1024)
+    f = cf.CuckooFilter(1024, config=c)
+    keys = O.gen_keys(41, 1080)
+    dk = _dev_keys(cf, keys)
+    for i in range(keys.size):
+        assert f.insertManySorted(dk[i:i + 1]) == of.insert_many_sorted(keys[i:i + 1])
+    np.testing.assert_array_equal(f.exportTable(), of.table_bytes())
+
+
+# ---------------------------------------------------------------------------
+# concurrent batches: invariants and binomial parity (2^20 slots, alpha 0.95)
+# ---------------------------------------------------------------------------
+
+def _check_invariants(O, oc, image, keys, occupied, nb):
+    tags = _tags(image, oc)
+    b, s = np.nonzero(tags)
+    stored = tags[b, s].astype(np.uint64)
+    assert stored.size == occupied
+    i1, i2, fp = O.candidates(oc, nb, keys)
+    fp2 = fp if oc.policy != 2 else np.array([O.alt_bucket(oc, nb, int(x), int(y))[1] for x, y in zip(i1, fp)],
+                                               np.uint32)
+    pairs = np.union1d((i1 << np.uint64(32)) | fp, (i2 << np.uint64(32)) | fp2)
+    assert np.isin((b.astype(np.uint64) << np.uint64(32)) | stored, pairs).all()
+    return stored, fp
+
+
+BATCH_CFGS = [
+    dict(eviction="bfs"),
+    dict(eviction="dfs"),
+    dict(eviction="bfs", block_size=64),
+    dict(eviction="bfs", block_size=512),
+    dict(eviction="bfs", word_bytes=4),
+    dict(eviction="bfs", bits_per_tag=8, word_bytes=4, bucket_size=16),
+    dict(eviction="bfs", bits_per_tag=32, bucket_size=8),
+    dict(eviction="bfs", policy="addsub"),
+    dict(eviction="dfs", policy="offset"),
+]
+
+
+@pytest.mark.parametrize("sorted_insert", [False, True], ids=["plain", "sorted"])
+@pytest.mark.parametrize("kw", BATCH_CFGS, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_batch_insert_invariants_and_binomial_parity(cf, oracle_mod, kw, sorted_insert):
+    O = oracle_mod
+    c = cf.make_config(**kw)
+    oc = _oracle_cfg(O, c)
+    cap = 1 << 20
+    of = O.Filter(oc, cap)
+    f = cf.CuckooFilter(cap, config=c)
+    nb = f.getNumBuckets()
+    n = int(f.capacity() * 0.95)
+    keys = O.gen_keys(42, n + (1 << 19))
+    pos, neg = keys[:n], keys[n:]
+    dpos = _dev_keys(cf, pos)
+    occ = f.insertManySorted(dpos) if sorted_insert else f.insertMany(dpos)
+    occ_cpu, ok_cpu = of.insert_many(pos)
+    image = f.exportTable()
+    assert occ == f.countOccupiedSlots()
+    stored, fp = _check_invariants(O, oc, image, pos, occ, nb)
+    fails_gpu, fails_cpu = n - occ, int((~ok_cpu).sum())
+    # failures: both essentially zero at 95% load; allow a binomial 99% band
+    band = 3 * np.sqrt(max(fails_cpu, 1)) + 3
+    assert abs(fails_gpu - fails_cpu) <= band, (fails_gpu, fails_cpu)
+    if fails_gpu == 0:
+        np.testing.assert_array_equal(np.sort(stored), np.sort(fp.astype(np.uint64)))
+        assert f.containsMany(dpos).all().item()
+    # FPR: two-sample binomial test against the oracle at the same load
+    fp_gpu = int(f.containsMany(_dev_keys(cf, neg)).sum().item())
+    fp_cpu = int(of.contains_many(neg).sum())
+    pooled = (fp_gpu + fp_cpu) / (2 * neg.size)
+    sd = np.sqrt(2 * neg.size * pooled * (1 - pooled)) + 1
+    assert abs(fp_gpu - fp_cpu) < 3.3 * sd, (fp_gpu, fp_cpu)
+    # delete half: every delete of an inserted key succeeds
+    if fails_gpu == 0:
+        out = __import__("torch").zeros(n // 2, dtype=__import__("torch").bool, device=dpos.device)
+        rem = f.deleteMany(dpos[: n // 2], out=out)
+        assert out.all().item() and rem == n - n // 2 == f.countOccupiedSlots()
+        assert f.containsMany(dpos[n // 2:]).all().item()
+    f.close()
+
+
+def test_eviction_histogram_accounts_for_every_insert(cf, oracle_mod):
+    c = cf.make_config(eviction="bfs", max_evictions=500)
+    f = cf.CuckooFilter(1 << 20, config=c)
+    f.setEvictionHistogram(True)
+    n = int(f.capacity() * 0.98)
+    keys = _dev_keys(cf, oracle_mod.gen_keys(5, n))
+    occ = f.insertMany(keys)
+    h = f.evictionHistogram()
+    fails = int(h[-1])
+    assert occ == n - fails
+    swaps = int(sum(i * int(v) for i, v in enumerate(h[:-1]))) + fails * 500
+    assert swaps == f.evictionCount()
+    assert h[1] > 0  # BFS moves happen at 98% load
+
+
+def test_empty_and_ragged_batches(cf, oracle_mod):
+    import torch
+
+    f = cf.CuckooFilter(10000, key_bytes=8, bits_per_tag=16)
+    empty = torch.empty(0, dtype=torch.int64, device="cuda")
+    assert f.insertMany(empty) == 0
+    assert f.insertManySorted(empty) == 0
+    assert f.deleteMany(empty) == 0
+    assert f.containsMany(empty).numel() == 0
+    for n in (1, 63, 64, 65, 255, 257, 1000):
+        keys = oracle_mod.gen_keys(100 + n, n)
+        f.clear()
+        assert f.insertMany(_dev_keys(cf, keys)) == n
+        assert f.containsMany(_dev_keys(cf, keys)).all().item()

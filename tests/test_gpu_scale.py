@@ -1,0 +1,45 @@
+"""Full-size properties at the HBM-scale workload: 2^28 slots (2^24 buckets x
+16 x 16-bit tags, 512 MiB), uint64 keys, BFS, filled to alpha = 0.95 with
+insertMany and with insertManySorted, then 2^27 lookups (50/50) and a 25%
+delete.  The oracle cannot replay 255M concurrent inserts, so the checks are
+size-independent: occupancy equals the non-empty slot count of the exported
+image, no inserted key is missing, FPR is within a binomial band of the
+closed form 2*b*alpha/2^f, every delete of an inserted key succeeds and the
+counter falls by exactly that many."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("sorted_insert", [False, True], ids=["insertMany", "insertManySorted"])
+def test_hbm_scale_fill_lookup_delete(cf, sorted_insert):
+    import torch
+    import workloads
+
+    f = cf.CuckooFilter(1 << 28, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    assert f.getNumBuckets() == 1 << 24 and f.sizeInBytes() == 512 << 20
+    n = int((1 << 28) * 0.95)
+    nq = 1 << 26
+    keys = workloads.gen_keys_torch(42, n + nq)
+    pos = keys[:n]
+    neg = keys[n:]
+    occ = f.insertManySorted(pos) if sorted_insert else f.insertMany(pos)
+    print(f"inserted {occ} of {n}", flush=True)
+    assert occ >= n - 10  # BFS+DFS(500) leaves essentially no failures at 95% load
+    img = f.exportTable()
+    assert int(np.count_nonzero(img.view(np.uint16))) == occ == f.countOccupiedSlots()
+    found = f.containsMany(pos)
+    missing = int((~found).sum().item())
+    assert missing <= n - occ
+    fps = int(f.containsMany(neg).sum().item())
+    p = 2 * 16 * (occ / (1 << 28)) / 65536
+    mu, sd = nq * p, np.sqrt(nq * p)
+    assert abs(fps - mu) < 4 * sd, (fps, mu)
+    if occ == n:
+        nd = n // 4
+        out = torch.zeros(nd, dtype=torch.bool, device=pos.device)
+        rem = f.deleteMany(pos[:nd], out=out)
+        assert out.all().item() and rem == n - nd
+        assert f.containsMany(pos[nd:]).all().item()
+    f.close()
