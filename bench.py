@@ -48,6 +48,10 @@ BYTES_PER_KEY = {
 }
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _events(torch):
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
@@ -98,6 +102,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sorted", action="store_true", help="also time insertManySorted")
+    ap.add_argument("--no-roofline", action="store_true", help="skip the measured gather/CAS roofline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the cuckoo-filter hot path has no multi-GPU leg in this build "
@@ -146,6 +151,7 @@ def main():
         checks["occupied"] = occ
         checks["remaining"] = rem
 
+    log(f"keys resident ({n_ins + N_LOOKUP // 2} u64); warmup {args.warmup}, timed {args.steps}")
     for _ in range(args.warmup):
         step(False)
     torch.cuda.synchronize()
@@ -182,6 +188,7 @@ def main():
         gbs = BYTES_PER_KEY[op] * units[op] / (avg[op] / 1e3) / 1e9
         per_op[f"{op}_algorithmic_gbs"] = round(gbs, 1)
 
+    log(f"plain steps done: {ms_step:.2f} ms/step")
     if args.sorted:
         ev = _events(torch)
         ts = []
@@ -222,10 +229,23 @@ def main():
         "checks": {"occupied_after_insert": checks.get("occupied"), "expected": n_ins,
                    "occupied_after_delete": checks.get("remaining")},
     }
-    if not args.no_cpu_baseline:
-        del keys, pos, query, dels
-        result["cpu_baseline"] = cpu_baseline(n_ins, N_LOOKUP, n_del)
+    del keys, pos, query, dels, out, dout
     f.close()
+    torch.cuda.empty_cache()
+    if not args.no_roofline:
+        import roofline as rl
+
+        log("measuring the random-gather roofline")
+        m = rl.measure()
+        result["measured_roofline"] = m
+        g = m.get("gather32_512MiB_gbs")
+        if g:
+            for op in ("insert", "lookup", "delete"):
+                result["breakdown"][f"{op}_frac_of_gather32"] = round(per_op[f"{op}_algorithmic_gbs"] / g, 3)
+            roofline["frac_of_measured_gather32"] = round(achieved / g, 3)
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline")
+        result["cpu_baseline"] = cpu_baseline(n_ins, N_LOOKUP, n_del)
     print(json.dumps(result), flush=True)
     return 0
 

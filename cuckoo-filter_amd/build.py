@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcuckoo_amd.so")
+RLIB = os.path.join(HERE, "libcuckoo_roofline.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CUCKOO_OFFLOAD_ARCH", "gfx950")
 
@@ -69,6 +70,25 @@ def build(force: bool = False, jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n$ {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # measurement kernels (random-gather / CAS / copy roofline), a separate library
+    robj = os.path.join(OBJ, "roofline.o")
+    if force or not os.path.exists(RLIB) or os.path.getmtime(RLIB) <= newest:
+        cmd = [HIPCC, *COMMON, "-shared", "-o", RLIB + ".tmp", os.path.join(CSRC, "roofline.hip")]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"roofline build failed:\n$ {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(RLIB + ".tmp", RLIB)
+    del robj
+    # C++ template-API test program (tests/cpp), run by the GPU test suite
+    src = os.path.join(HERE, "..", "tests", "cpp", "test_cuckoo_filter.cpp")
+    exe = os.path.splitext(src)[0]
+    if os.path.exists(src) and (force or not os.path.exists(exe) or os.path.getmtime(exe) <= max(newest, os.path.getmtime(src))):
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-std=c++20", "-O2", "-Xclang", "-target-feature", "-Xclang",
+               "-packed-fp32-ops", f"-I{CSRC}", src, "-o", exe + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ API test build failed:\n$ {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 

@@ -90,47 +90,49 @@ __device__ __forceinline__ Word casRelaxed(Word* p, Word expected, Word desired)
     return expected;
 }
 
-/// Register copy of one bucket.
+/// Register copy of one bucket: a vector value, passed and returned by
+/// value, so that the dynamically indexed word accesses below stay in
+/// registers (v_cndmask selects).  A bucket kept in an addressable array (or a
+/// struct passed by reference) gets its indexed reads rewritten into memory
+/// loads with a computed offset, which moves the bucket to scratch or LDS.
 template <typename Word, size_t WC>
-struct Snap {
-    Word w[WC];
-};
+using Snap = Word __attribute__((ext_vector_type(WC)));
 
 /// Loads a whole bucket with the widest aligned vector loads it allows.
 template <typename Word, size_t WC>
 __device__ __forceinline__ Snap<Word, WC> loadBucket(const Word* p) {
     constexpr size_t bytes = WC * sizeof(Word);
-    Snap<Word, WC> s;
     if constexpr (bytes % 16 == 0) {
         using V = unsigned int __attribute__((ext_vector_type(4)));
-        V v[bytes / 16];
+        using U = unsigned int __attribute__((ext_vector_type(bytes / 4)));
+        U u;
 #pragma unroll
-        for (size_t i = 0; i < bytes / 16; ++i) v[i] = reinterpret_cast<const V*>(p)[i];
-        __builtin_memcpy(s.w, v, bytes);
+        for (size_t i = 0; i < bytes / 16; ++i) {
+            const V v = reinterpret_cast<const V*>(p)[i];
+            u[4 * i + 0] = v.x;
+            u[4 * i + 1] = v.y;
+            u[4 * i + 2] = v.z;
+            u[4 * i + 3] = v.w;
+        }
+        return __builtin_bit_cast(Snap<Word, WC>, u);
     } else if constexpr (bytes == 8) {
         using V = unsigned int __attribute__((ext_vector_type(2)));
-        V v = *reinterpret_cast<const V*>(p);
-        __builtin_memcpy(s.w, &v, 8);
+        return __builtin_bit_cast(Snap<Word, WC>, *reinterpret_cast<const V*>(p));
     } else {
         static_assert(bytes == 4, "bucket of unexpected size");
-        unsigned int v = *reinterpret_cast<const unsigned int*>(p);
-        __builtin_memcpy(s.w, &v, 4);
+        return __builtin_bit_cast(Snap<Word, WC>, *reinterpret_cast<const unsigned int*>(p));
     }
+}
+
+template <typename Word, size_t WC>
+__device__ __forceinline__ Word pick(Snap<Word, WC> s, uint32_t idx) {
+    return s[idx];
+}
+
+template <typename Word, size_t WC>
+__device__ __forceinline__ Snap<Word, WC> put(Snap<Word, WC> s, uint32_t idx, Word v) {
+    s[idx] = v;
     return s;
-}
-
-template <typename Word, size_t WC>
-__device__ __forceinline__ Word pick(const Snap<Word, WC>& s, uint32_t idx) {
-    Word r = s.w[0];
-#pragma unroll
-    for (uint32_t j = 1; j < WC; ++j) r = (idx == j) ? s.w[j] : r;
-    return r;
-}
-
-template <typename Word, size_t WC>
-__device__ __forceinline__ void put(Snap<Word, WC>& s, uint32_t idx, Word v) {
-#pragma unroll
-    for (uint32_t j = 0; j < WC; ++j) s.w[j] = (idx == j) ? v : s.w[j];
 }
 
 /// First word index at or after `start` (cyclically) whose bit is set in `mask`.
@@ -196,31 +198,31 @@ struct FilterOps {
     // snapshot primitives
     // ---------------------------------------------------------------------
 
-    static __device__ __forceinline__ uint64_t emptyWords(const SnapT& s) {
+    static __device__ __forceinline__ uint64_t emptyWords(SnapT s) {
         uint64_t m = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < WC; ++j) m |= uint64_t(getZeroMask<Tag, Word>(s.w[j]) != 0) << j;
+        for (uint32_t j = 0; j < WC; ++j) m |= uint64_t(getZeroMask<Tag, Word>(s[j]) != 0) << j;
         return m;
     }
 
-    static __device__ __forceinline__ uint64_t matchWords(const SnapT& s, Tag tag) {
+    static __device__ __forceinline__ uint64_t matchWords(SnapT s, Tag tag) {
         const Word rep = replicateTag<Tag, Word>(tag);
         uint64_t m = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < WC; ++j) m |= uint64_t(getZeroMask<Tag, Word>(s.w[j] ^ rep) != 0) << j;
+        for (uint32_t j = 0; j < WC; ++j) m |= uint64_t(getZeroMask<Tag, Word>(s[j] ^ rep) != 0) << j;
         return m;
     }
 
-    static __device__ __forceinline__ bool snapHas(const SnapT& s, Tag tag) {
+    static __device__ __forceinline__ bool snapHas(SnapT s, Tag tag) {
         const Word rep = replicateTag<Tag, Word>(tag);
         Word acc = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < WC; ++j) acc |= getZeroMask<Tag, Word>(s.w[j] ^ rep);
+        for (uint32_t j = 0; j < WC; ++j) acc |= getZeroMask<Tag, Word>(s[j] ^ rep);
         return acc != 0;
     }
 
     /// tryInsertAtBucket on a snapshot the caller already holds (CuckooFilter.cuh:859-904).
-    static __device__ __forceinline__ bool tryInsertSnap(Word* bw, SnapT& s, Tag tag) {
+    static __device__ __forceinline__ bool tryInsertSnap(Word* bw, SnapT s, Tag tag) {
         const uint32_t start = startWord(tag);
         for (;;) {
             const uint64_t m = emptyWords(s);
@@ -230,12 +232,12 @@ struct FilterOps {
             const uint32_t lane = lowestBit(getZeroMask<Tag, Word>(cur)) / BITS;
             const Word old = detail::casRelaxed(bw + w, cur, replaceTag(cur, lane, tag));
             if (old == cur) return true;
-            detail::put(s, w, old);
+            s = detail::put(s, w, old);
         }
     }
 
     /// tryRemoveAtBucket on a snapshot (CuckooFilter.cuh:805-848).
-    static __device__ __forceinline__ bool tryRemoveSnap(Word* bw, SnapT& s, Tag tag) {
+    static __device__ __forceinline__ bool tryRemoveSnap(Word* bw, SnapT s, Tag tag) {
         const uint32_t start = startWord(tag);
         const Word rep = replicateTag<Tag, Word>(tag);
         for (;;) {
@@ -246,7 +248,7 @@ struct FilterOps {
             const uint32_t lane = lowestBit(getZeroMask<Tag, Word>(cur ^ rep)) / BITS;
             const Word old = detail::casRelaxed(bw + w, cur, replaceTag(cur, lane, EMPTY));
             if (old == cur) return true;
-            detail::put(s, w, old);
+            s = detail::put(s, w, old);
         }
     }
 
@@ -257,8 +259,7 @@ struct FilterOps {
     static __device__ __forceinline__ bool tryInsertAtBucket(const View& v, size_t b, Tag tag) {
         Word* bw = bucketPtr(v, b);
         if constexpr (kSnapshot) {
-            SnapT s = detail::loadBucket<Word, WC>(bw);
-            return tryInsertSnap(bw, s, tag);
+            return tryInsertSnap(bw, detail::loadBucket<Word, WC>(bw), tag);
         } else {
             const uint32_t start = startWord(tag);
             for (uint32_t i = 0; i < WC; ++i) {
@@ -280,8 +281,7 @@ struct FilterOps {
     static __device__ __forceinline__ bool tryRemoveAtBucket(const View& v, size_t b, Tag tag) {
         Word* bw = bucketPtr(v, b);
         if constexpr (kSnapshot) {
-            SnapT s = detail::loadBucket<Word, WC>(bw);
-            return tryRemoveSnap(bw, s, tag);
+            return tryRemoveSnap(bw, detail::loadBucket<Word, WC>(bw), tag);
         } else {
             const uint32_t start = startWord(tag);
             const Word rep = replicateTag<Tag, Word>(tag);
@@ -313,8 +313,7 @@ struct FilterOps {
             const uint32_t c0 = startWord(tag) / CW;
             for (uint32_t i = 0; i < NCH; ++i) {
                 const uint32_t c = (c0 + i) & (NCH - 1);
-                if (detail::Snap<Word, CW> s = detail::loadBucket<Word, CW>(bw + c * CW);
-                    FilterOps::chunkHas<CW>(s, tag)) {
+                if (FilterOps::chunkHas<CW>(detail::loadBucket<Word, CW>(bw + c * CW), tag)) {
                     return true;
                 }
             }
@@ -323,11 +322,11 @@ struct FilterOps {
     }
 
     template <size_t N>
-    static __device__ __forceinline__ bool chunkHas(const detail::Snap<Word, N>& s, Tag tag) {
+    static __device__ __forceinline__ bool chunkHas(detail::Snap<Word, N> s, Tag tag) {
         const Word rep = replicateTag<Tag, Word>(tag);
         Word acc = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < N; ++j) acc |= getZeroMask<Tag, Word>(s.w[j] ^ rep);
+        for (uint32_t j = 0; j < N; ++j) acc |= getZeroMask<Tag, Word>(s[j] ^ rep);
         return acc != 0;
     }
 

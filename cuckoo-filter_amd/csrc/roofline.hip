@@ -1,0 +1,122 @@
+// Measurement kernels for the roofline that bounds the cuckoo filter on one
+// MI355X (built into libcuckoo_roofline.so; no part of the filter library):
+//
+//  * rl_gather: per key, read `gathers` uniformly random, aligned blocks of
+//    `gather_bytes` (32 or 64) from a table, write one byte.  The same key
+//    stream in and byte out as containsMany, with the hashing reduced to one
+//    multiply so the memory system is what is measured.  `dependent` makes
+//    each further block's address depend on the data of the previous one
+//    (the shape of a lookup's second probe).
+//  * rl_cas: per key, one relaxed device-scope 64-bit CAS at a random word
+//    (the write half of an insert).
+//  * rl_copy: a plain streaming copy, used to calibrate the FETCH_SIZE /
+//    WRITE_SIZE counters against a known byte count.
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace {
+
+__device__ __forceinline__ uint64_t mix(uint64_t z) {
+    z ^= z >> 31;
+    z *= 0x9E3779B97F4A7C15ull;
+    return z ^ (z >> 29);
+}
+
+using V4 = unsigned int __attribute__((ext_vector_type(4)));
+
+template <int GB>
+__device__ __forceinline__ unsigned int readBlock(const V4* p) {
+    V4 acc = p[0];
+#pragma unroll
+    for (int i = 1; i < GB / 16; ++i) acc ^= p[i];
+    return acc.x ^ acc.y ^ acc.z ^ acc.w;
+}
+
+template <int GB>
+__global__ __launch_bounds__(1024) void gatherKernel(const V4* __restrict__ table, uint64_t blockMask,
+                                                     const uint64_t* __restrict__ keys, size_t n,
+                                                     uint8_t* __restrict__ out, int gathers, int dependent) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i];
+        uint64_t h = mix(k);
+        unsigned int acc = readBlock<GB>(table + (h & blockMask) * (GB / 16));
+        for (int g = 1; g < gathers; ++g) {
+            h = mix(h + g + (dependent ? acc : 0u));
+            acc ^= readBlock<GB>(table + (h & blockMask) * (GB / 16));
+        }
+        out[i] = acc == 0x5A5A5A5Au;
+    }
+}
+
+__global__ __launch_bounds__(1024) void casKernel(unsigned long long* __restrict__ words, uint64_t wordMask,
+                                                  const uint64_t* __restrict__ keys, size_t n) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i];
+        unsigned long long* p = words + (mix(k) & wordMask);
+        unsigned long long expected = 0;
+        __hip_atomic_compare_exchange_strong(p, &expected, k | 1ull, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ __launch_bounds__(1024) void copyKernel(const V4* __restrict__ src, V4* __restrict__ dst, size_t n16) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+uint32_t gridFor(const void* k, size_t n, int block) {
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, block, 0);
+    if (per < 1) per = 1;
+    const size_t need = (n + block - 1) / block;
+    const size_t cap = size_t(per) * size_t(cus > 0 ? cus : 1);
+    return uint32_t(need < cap ? (need ? need : 1) : cap);
+}
+
+bool pow2(uint64_t v) { return v && !(v & (v - 1)); }
+
+}  // namespace
+
+extern "C" {
+
+// Returns 0 on success, -1 on a bad argument, else the HIP error code.
+int rl_gather(const void* table, size_t table_bytes, int gather_bytes, const uint64_t* keys, size_t n,
+              uint8_t* out, int gathers, int dependent, int block, void* stream) {
+    if (!table || !keys || !out || (gather_bytes != 32 && gather_bytes != 64) || gathers < 1 || block < 64 ||
+        block > 1024 || table_bytes % gather_bytes || !pow2(table_bytes / gather_bytes))
+        return -1;
+    const uint64_t mask = table_bytes / gather_bytes - 1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (gather_bytes == 32) {
+        gatherKernel<32><<<gridFor((const void*)gatherKernel<32>, n, block), block, 0, s>>>(
+            static_cast<const V4*>(table), mask, keys, n, out, gathers, dependent);
+    } else {
+        gatherKernel<64><<<gridFor((const void*)gatherKernel<64>, n, block), block, 0, s>>>(
+            static_cast<const V4*>(table), mask, keys, n, out, gathers, dependent);
+    }
+    return static_cast<int>(hipGetLastError());
+}
+
+int rl_cas(void* table, size_t table_bytes, const uint64_t* keys, size_t n, int block, void* stream) {
+    if (!table || !keys || block < 64 || block > 1024 || table_bytes % 8 || !pow2(table_bytes / 8)) return -1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    casKernel<<<gridFor((const void*)casKernel, n, block), block, 0, s>>>(
+        static_cast<unsigned long long*>(table), table_bytes / 8 - 1, keys, n);
+    return static_cast<int>(hipGetLastError());
+}
+
+int rl_copy(const void* src, void* dst, size_t bytes, int block, void* stream) {
+    if (!src || !dst || bytes % 16 || block < 64 || block > 1024) return -1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    copyKernel<<<gridFor((const void*)copyKernel, bytes / 16, block), block, 0, s>>>(
+        static_cast<const V4*>(src), static_cast<V4*>(dst), bytes / 16);
+    return static_cast<int>(hipGetLastError());
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+"""Measured memory roofline for the cuckoo-filter kernels on one GPU.
+
+Runs the kernels of libcuckoo_roofline.so with HIP events:
+  gather32 / gather64 : uniformly random aligned 32-/64-byte block reads,
+                        8-byte key in and 1-byte result out per key, over a
+                        table of the filter's size (512 MiB) and 4x that;
+  gather32x2 dep      : two dependent 32-byte reads per key (lookup miss shape);
+  cas8                : one random 64-bit CAS per key;
+  copy                : streaming copy (HBM bandwidth reference).
+Each figure is algorithmic bytes (the bytes named above) per second.
+
+    python cuckoo-filter_amd/roofline.py            # prints one JSON object
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import sys
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+RLIB_PATH = os.path.join(_HERE, "libcuckoo_roofline.so")
+
+
+def _lib():
+    if not os.path.exists(RLIB_PATH):
+        raise ImportError(f"{RLIB_PATH} is not built; run python cuckoo-filter_amd/build.py")
+    L = C.CDLL(RLIB_PATH)
+    P, sz, i = C.c_void_p, C.c_size_t, C.c_int
+    L.rl_gather.argtypes = [P, sz, i, P, sz, P, i, i, i, P]
+    L.rl_cas.argtypes = [P, sz, P, sz, i, P]
+    L.rl_copy.argtypes = [P, P, sz, i, P]
+    for f in (L.rl_gather, L.rl_cas, L.rl_copy):
+        f.restype = i
+    return L
+
+
+def _time(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps / 1e3
+
+
+def measure(table_bytes=512 << 20, n=1 << 26, reps=5, block=256, big_factor=4) -> dict:
+    import torch
+
+    sys.path.insert(0, _HERE)
+    import workloads
+
+    L = _lib()
+    dev = torch.device("cuda:0")
+    s = C.c_void_p(int(torch.cuda.current_stream().cuda_stream))
+    keys = workloads.gen_keys_torch(7, n, device=dev)
+    out = torch.empty(n, dtype=torch.uint8, device=dev)
+    res = {"keys": n, "block": block}
+
+    def gather(tab, tb, gb, g, dep):
+        def run():
+            rc = L.rl_gather(C.c_void_p(tab.data_ptr()), tb, gb, C.c_void_p(keys.data_ptr()), n,
+                             C.c_void_p(out.data_ptr()), g, dep, block, s)
+            assert rc == 0, rc
+        t = _time(torch, run, reps)
+        return (8 + 1 + g * gb) * n / t / 1e9
+
+    for factor in (1, big_factor):
+        tb = table_bytes * factor
+        tab = torch.randint(0, 2**31 - 1, (tb // 4,), dtype=torch.int32, device=dev)
+        tag = f"{tb >> 20}MiB"
+        res[f"gather32_{tag}_gbs"] = round(gather(tab, tb, 32, 1, 0), 1)
+        res[f"gather64_{tag}_gbs"] = round(gather(tab, tb, 64, 1, 0), 1)
+        res[f"gather32x2dep_{tag}_gbs"] = round(gather(tab, tb, 32, 2, 1), 1)
+
+        def cas():
+            rc = L.rl_cas(C.c_void_p(tab.data_ptr()), tb, C.c_void_p(keys.data_ptr()), n, block, s)
+            assert rc == 0, rc
+        t = _time(torch, cas, reps)
+        res[f"cas8_{tag}_gbs"] = round((8 + 8) * n / t / 1e9, 1)
+        res[f"cas8_{tag}_gops"] = round(n / t / 1e9, 2)
+        del tab
+        torch.cuda.empty_cache()
+
+    cb = 1 << 30
+    src = torch.empty(cb // 4, dtype=torch.int32, device=dev)
+    dst = torch.empty_like(src)
+
+    def copy():
+        rc = L.rl_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), cb, block, s)
+        assert rc == 0, rc
+    t = _time(torch, copy, reps)
+    res["copy_gbs"] = round(2 * cb / t / 1e9, 1)
+    return res
+
+
+if __name__ == "__main__":
+    print(json.dumps(measure()), flush=True)
