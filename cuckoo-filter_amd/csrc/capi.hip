@@ -242,7 +242,7 @@ int cf_insert_many(cf_filter* f, const void* d_keys, size_t n, void* stream, siz
     if (r != CF_OK) return r;
     DeviceGuard g(f->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipError_t e = f->ops->insert(f->view(), d_keys, n, f->cfg.block_size, s);
+    hipError_t e = f->ops->insert(f->view(), d_keys, n, f->cfg.block_size, s, f->ws);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hipFail(e, "insertMany");
     return readOccupied(f, s, occupied);

@@ -63,8 +63,9 @@ struct Entry {
         }
     }
 
-    static hipError_t insert(const cfabi::AnyView& a, const void* keys, size_t n, uint32_t block, hipStream_t s) {
-        return E::insert(view(a), static_cast<const Key*>(keys), n, block, s);
+    static hipError_t insert(const cfabi::AnyView& a, const void* keys, size_t n, uint32_t block, hipStream_t s,
+                             cuckoo::Workspace& ws) {
+        return E::insert(view(a), static_cast<const Key*>(keys), n, block, s, ws);
     }
     static hipError_t insertSorted(const cfabi::AnyView& a, const void* keys, size_t n, uint32_t block,
                                    hipStream_t s, cuckoo::Workspace& ws) {

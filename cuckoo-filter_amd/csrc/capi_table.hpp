@@ -8,7 +8,7 @@
 #include <cstddef>
 #include <cstdint>
 
-#include "cuckoo/kernels.hpp"
+#include "cuckoo/engine.hpp"
 
 namespace cfabi {
 
@@ -28,7 +28,8 @@ struct Ops {
     size_t bucketBytes;
     size_t (*numBuckets)(size_t capacity);
     void (*hostCandidates)(size_t nb, const void* keys, size_t n, uint64_t* i1, uint64_t* i2, uint32_t* fp);
-    hipError_t (*insert)(const AnyView&, const void* keys, size_t n, uint32_t block, hipStream_t);
+    hipError_t (*insert)(const AnyView&, const void* keys, size_t n, uint32_t block, hipStream_t,
+                         cuckoo::Workspace&);
     hipError_t (*insertSorted)(const AnyView&, const void* keys, size_t n, uint32_t block, hipStream_t,
                                cuckoo::Workspace&);
     hipError_t (*contains)(const AnyView&, const void* keys, size_t n, bool* out, uint32_t block, hipStream_t);

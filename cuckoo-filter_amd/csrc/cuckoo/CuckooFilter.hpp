@@ -34,6 +34,7 @@
 #include "device_ops.hpp"
 #include "hashutil.hpp"
 #include "helpers.hpp"
+#include "engine.hpp"
 #include "kernels.hpp"
 
 #if !defined(CUCKOO_FILTER_NO_THRUST) && __has_include(<thrust/device_vector.h>)
@@ -221,7 +222,7 @@ struct CuckooFilter {
 
     /// Inserts n keys; returns the total number of occupied slots afterwards.
     size_t insertMany(const T* d_keys, const size_t n, hipStream_t stream = {}) {
-        HIP_CALL((cuckoo::Engine<Config>::insert(view(), d_keys, n, blockSize, stream)));
+        HIP_CALL((cuckoo::Engine<Config>::insert(view(), d_keys, n, blockSize, stream, workspace_)));
         HIP_CALL(hipStreamSynchronize(stream));
         return occupiedSlots();
     }

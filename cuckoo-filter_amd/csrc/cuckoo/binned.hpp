@@ -1,0 +1,304 @@
+// Bucket-binned insert: the MI355X path for large insert batches.
+//
+// A device-wide CAS on MI355X is executed at the memory side and moves a
+// 64-byte line; random CAS tops out near 22 G/s, and the reference's
+// one-thread-per-key insert (CuckooFilter.cuh:1119-1141) issues at least one
+// per key.  This path removes the CAS from the common case:
+//
+//  1. entries (primary bucket << 32 | fingerprint) are computed from the keys
+//     inside the first pass of an LSD radix partition that groups them by
+//     table slice (a slice is the run of buckets that fills kSliceBytes);
+//  2. one workgroup per slice copies the slice into LDS, performs every
+//     primary-bucket attempt of its entries there (the same
+//     tryInsertAtBucket slot choice, with LDS compare-and-swap between the
+//     workgroup's own lanes) and writes the slice back with plain vector
+//     stores; entries whose primary bucket is full are appended to an
+//     overflow list;
+//  3. the overflow entries continue exactly where the reference's insert
+//     continues after a full primary bucket -- the secondary bucket, then
+//     BFS/DFS eviction -- with device-scope CAS.
+//
+// Why this is the reference's method: its insert is concurrent, so every
+// interleaving of the per-key steps is a possible outcome.  Running all
+// primary-bucket attempts first, then all the rest, is one such
+// interleaving, and during step 2 no other workgroup touches a slice while it
+// sits in LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+
+#include "device_ops.hpp"
+#include "kernels.hpp"
+#include "radix.hpp"
+
+namespace cuckoo {
+
+/// Table bytes one workgroup stages in LDS.
+inline constexpr size_t kSliceBytes = 32 * 1024;
+inline constexpr int kBinBlock = 1024;
+
+/// Packed (primary bucket, fingerprint) entry of key i.  With SortedSemantics
+/// the primary bucket is the one insertManySorted derives from the reference's
+/// PackedTagType (CuckooFilter.cuh:159,1205-1206), which keeps only the bucket
+/// bits that fit beside the tag.
+template <typename Config, bool SortedSemantics>
+struct EntrySource {
+    using Key = typename Config::KeyType;
+    const Key* keys;
+    size_t numBuckets;
+
+    __device__ __forceinline__ uint64_t operator()(size_t i) const {
+        const auto c = Config::AltBucketPolicy::getCandidateBuckets(keys[i], numBuckets);
+        uint64_t i1 = c.i1;
+        if constexpr (SortedSemantics) {
+            using P = std::conditional_t<Config::bitsPerTag <= 8, uint32_t, uint64_t>;
+            i1 = static_cast<uint64_t>(static_cast<P>(static_cast<P>(c.i1) << Config::bitsPerTag) >>
+                                       Config::bitsPerTag);
+        }
+        return (i1 << 32) | static_cast<uint64_t>(c.fp);
+    }
+};
+
+template <typename Src>
+__global__ __launch_bounds__(1024) void packEntriesKernel(Src src, size_t n, uint64_t* __restrict__ out) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = src(i);
+}
+
+/// start[s] = first entry whose slice is >= s (entries sorted by slice), for
+/// s in [0, numSlices]; also zeroes the overflow counter.
+template <typename T = uint64_t>
+__global__ __launch_bounds__(256) void sliceStartKernel(const T* __restrict__ entries, uint32_t n, int sliceShift,
+                                                        uint32_t numSlices, uint32_t* __restrict__ start,
+                                                        unsigned long long* overflowCount) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) *overflowCount = 0;
+    if (s > numSlices) return;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if ((entries[mid] >> sliceShift) < s) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    start[s] = lo;
+}
+
+/// Overflow entries a workgroup buffers in LDS before reserving their global
+/// space with a single atomic (more than this spill with one atomic per wave).
+inline constexpr uint32_t kOverflowLds = 2048;
+
+/// Dynamic LDS of binInsertKernel: the table slice, then the overflow buffer.
+inline constexpr size_t binOverflowOffset(size_t sliceBytes) { return (sliceBytes + 15) / 16 * 16; }
+inline size_t binLdsBytes(size_t sliceBytes) { return binOverflowOffset(sliceBytes) + kOverflowLds * sizeof(uint64_t); }
+
+/// Step 2: primary-bucket attempts of one slice's entries inside LDS.
+template <typename Config>
+__global__ __launch_bounds__(kBinBlock) void binInsertKernel(const uint64_t* __restrict__ entries,
+                                                             const uint32_t* __restrict__ start, int log2Slice,
+                                                             DeviceView<Config> v, uint64_t* __restrict__ overflow,
+                                                             unsigned long long* overflowCount) {
+    using Ops = FilterOps<Config>;
+    using Word = typename Config::WordType;
+    using Tag = typename Config::TagType;
+    constexpr size_t WC = Layout<Config>::wordCount;
+    constexpr size_t kBucketBytes = Layout<Config>::bucketBytes;
+    extern __shared__ __align__(16) unsigned char ldsRaw[];
+    __shared__ uint32_t ovCount;
+    __shared__ unsigned long long ovBase;
+
+    const uint32_t s = blockIdx.x;
+    const uint32_t lo = start[s];
+    const uint32_t hi = start[s + 1];
+    if (lo == hi) return;  // no entry for this slice: the table is left untouched
+
+    const size_t sliceBuckets = static_cast<size_t>(1) << log2Slice;
+    Word* slice = reinterpret_cast<Word*>(ldsRaw);
+    uint64_t* ovBuf = reinterpret_cast<uint64_t*>(ldsRaw + binOverflowOffset(sliceBuckets * kBucketBytes));
+    const size_t b0 = static_cast<size_t>(s) << log2Slice;
+    const size_t nbs = v.numBuckets - b0 < sliceBuckets ? v.numBuckets - b0 : sliceBuckets;
+    const size_t words = nbs * WC;
+    const Word* g = v.words + b0 * WC;
+    Word* gw = v.words + b0 * WC;
+
+    if (threadIdx.x == 0) ovCount = 0;
+    using V4 = unsigned int __attribute__((ext_vector_type(4)));
+    const size_t bytes = words * sizeof(Word);
+    const size_t vecs = bytes / 16;
+    for (size_t i = threadIdx.x; i < vecs; i += kBinBlock) {
+        reinterpret_cast<V4*>(slice)[i] = reinterpret_cast<const V4*>(g)[i];
+    }
+    for (size_t i = vecs * 16 / sizeof(Word) + threadIdx.x; i < words; i += kBinBlock) slice[i] = g[i];
+    __syncthreads();
+
+    uint32_t ok = 0;
+    const int lane = __lane_id();
+    const uint64_t ltMask = (1ull << lane) - 1;
+    for (uint32_t e = lo + threadIdx.x; e < hi; e += kBinBlock) {
+        const uint64_t entry = entries[e];
+        const size_t local = static_cast<size_t>(entry >> 32) - b0;
+        const Tag fp = static_cast<Tag>(entry);
+        Word* bw = slice + local * WC;
+        const bool placed = Ops::tryInsertSnap(bw, detail::loadBucket<Word, WC>(bw), fp);
+        ok += placed;
+        const uint64_t fails = __ballot(!placed);
+        if (fails != 0) {
+            // one LDS atomic per wave reserves this wave's run in the workgroup buffer
+            const int leader = __ffsll(static_cast<long long>(fails)) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&ovCount, static_cast<uint32_t>(__popcll(fails)));
+            base = __shfl(base, leader, 64);
+            const uint32_t mine = base + static_cast<uint32_t>(__popcll(fails & ltMask));
+            if (!placed) {
+                if (mine < kOverflowLds) {
+                    ovBuf[mine] = entry;
+                } else {
+                    overflow[atomicAdd(overflowCount, 1ull)] = entry;  // buffer full (rare)
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t buffered = ovCount < kOverflowLds ? ovCount : kOverflowLds;
+    if (threadIdx.x == 0 && buffered != 0) ovBase = atomicAdd(overflowCount, static_cast<unsigned long long>(buffered));
+    for (size_t i = threadIdx.x; i < vecs; i += kBinBlock) {
+        reinterpret_cast<V4*>(gw)[i] = reinterpret_cast<const V4*>(slice)[i];
+    }
+    for (size_t i = vecs * 16 / sizeof(Word) + threadIdx.x; i < words; i += kBinBlock) gw[i] = slice[i];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < buffered; i += kBinBlock) overflow[ovBase + i] = ovBuf[i];
+    detail::blockCountAdd(ok, v.occupied, false);
+}
+
+/// Step 3: the overflow entries continue from their secondary bucket.
+template <typename Config>
+__global__ __launch_bounds__(kMaxBlock) void insertOverflowKernel(const uint64_t* __restrict__ overflow,
+                                                                  const unsigned long long* __restrict__ count,
+                                                                  DeviceView<Config> v) {
+    using Ops = FilterOps<Config>;
+    using Tag = typename Config::TagType;
+    const size_t n = static_cast<size_t>(*count);
+    uint32_t ok = 0;
+    uint32_t ev = 0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t entry = overflow[i];
+        uint32_t chain;
+        const bool s = Ops::insertAfterPrimary(v, static_cast<size_t>(entry >> 32), static_cast<Tag>(entry), chain);
+        ok += s;
+        ev += chain;
+        if (v.chainHist != nullptr && (chain != 0 || !s)) {
+            atomicAdd(&v.chainHist[s ? chain : v.maxEvictions + 1], 1u);
+        }
+    }
+    detail::blockCountAdd(ok, v.occupied, false);
+    if (v.evictions != nullptr) detail::blockCountAdd(ev, v.evictions, false);
+}
+
+namespace detail {
+
+inline int log2Floor(size_t v) {
+    int b = 0;
+    while ((size_t(2) << b) <= v) ++b;
+    return b;
+}
+
+/// Insert path selection: CUCKOO_INSERT_PATH = "direct" | "binned" | "auto"
+/// (default auto: binned once the batch has at least one key per
+/// kBinnedMinBucketsPerKey buckets).
+enum class InsertPath { Auto, Direct, Binned };
+
+inline InsertPath insertPathFromEnv() {
+    const char* e = std::getenv("CUCKOO_INSERT_PATH");
+    if (e == nullptr) return InsertPath::Auto;
+    if (std::strcmp(e, "direct") == 0) return InsertPath::Direct;
+    if (std::strcmp(e, "binned") == 0) return InsertPath::Binned;
+    return InsertPath::Auto;
+}
+
+inline constexpr size_t kBinnedMinBucketsPerKey = 4;
+
+}  // namespace detail
+
+/// Host orchestration of the binned insert for one configuration.
+template <typename Config>
+struct BinnedInsert {
+    using Key = typename Config::KeyType;
+    using View = DeviceView<Config>;
+    static constexpr size_t kBucketBytes = Layout<Config>::bucketBytes;
+
+    /// Whether the table can be processed by this path (32-bit bucket field).
+    static bool applicable(size_t numBuckets) {
+        return numBuckets <= (size_t(1) << 32) && kBucketBytes <= kSliceBytes;
+    }
+
+    static int log2Slice(size_t numBuckets) {
+        int l = detail::log2Floor(kSliceBytes / kBucketBytes);
+        const int nbBits = detail::bitsFor(numBuckets);
+        return l < nbBits ? l : nbBits;
+    }
+
+    static size_t numSlices(size_t numBuckets) {
+        const size_t S = size_t(1) << log2Slice(numBuckets);
+        return (numBuckets + S - 1) / S;
+    }
+
+    static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+    static size_t workspaceBytes(size_t n, size_t numBuckets) {
+        return 2 * align256(n * sizeof(uint64_t)) + align256(radix::scratchBytes(n)) +
+               align256((numSlices(numBuckets) + 1) * sizeof(uint32_t)) + 256;
+    }
+
+    template <bool SortedSemantics>
+    static hipError_t run(const View& v, const Key* keys, size_t n, uint32_t block, hipStream_t s, Workspace& ws) {
+        constexpr size_t kChunk = size_t(1) << 31;
+        const int l2s = log2Slice(v.numBuckets);
+        const size_t slices = numSlices(v.numBuckets);
+        const int sliceBits = detail::bitsFor(slices);
+        for (size_t off = 0; off < n; off += kChunk) {
+            const size_t m = n - off < kChunk ? n - off : kChunk;
+            hipError_t e = ws.reserve(workspaceBytes(m, v.numBuckets));
+            if (e != hipSuccess) return e;
+            char* p = static_cast<char*>(ws.ptr);
+            uint64_t* a = reinterpret_cast<uint64_t*>(p);
+            p += align256(m * sizeof(uint64_t));
+            uint64_t* b = reinterpret_cast<uint64_t*>(p);
+            p += align256(m * sizeof(uint64_t));
+            void* scratch = p;
+            p += align256(radix::scratchBytes(m));
+            uint32_t* start = reinterpret_cast<uint32_t*>(p);
+            p += align256((slices + 1) * sizeof(uint32_t));
+            unsigned long long* overflowCount = reinterpret_cast<unsigned long long*>(p);
+
+            EntrySource<Config, SortedSemantics> src{keys + off, v.numBuckets};
+            uint64_t* sorted = a;
+            if (sliceBits > 0) {
+                e = radix::sortBits<uint64_t>(src, m, 32 + l2s, 32 + l2s + sliceBits, a, b, scratch, s, &sorted);
+                if (e != hipSuccess) return e;
+            } else {
+                auto k = packEntriesKernel<decltype(src)>;
+                k<<<detail::gridFor(k, m, 256), 256, 0, s>>>(src, m, a);
+            }
+            uint64_t* overflow = sorted == a ? b : a;
+            sliceStartKernel<uint64_t><<<static_cast<uint32_t>((slices + 1 + 255) / 256), 256, 0, s>>>(
+                sorted, static_cast<uint32_t>(m), 32 + l2s, static_cast<uint32_t>(slices), start, overflowCount);
+            const size_t lds = binLdsBytes((size_t(1) << l2s) * kBucketBytes);
+            binInsertKernel<Config><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(sorted, start, l2s, v,
+                                                                                       overflow, overflowCount);
+            auto k = insertOverflowKernel<Config>;
+            k<<<detail::gridFor(k, m, block), block, 0, s>>>(overflow, overflowCount, v);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+};
+
+}  // namespace cuckoo

@@ -400,6 +400,14 @@ struct FilterOps {
     static __device__ __forceinline__ bool insertFrom(const View& v, size_t i1, Tag fp, uint32_t& chain) {
         chain = 0;
         if (tryInsertAtBucket(v, i1, fp)) return true;
+        return insertAfterPrimary(v, i1, fp, chain);
+    }
+
+    /// The rest of an insert whose attempt at the primary bucket i1 has
+    /// already failed: the secondary bucket, then eviction from the bucket
+    /// the fingerprint's low bit selects (CuckooFilter.cuh:1062-1076).
+    static __device__ __forceinline__ bool insertAfterPrimary(const View& v, size_t i1, Tag fp, uint32_t& chain) {
+        chain = 0;
         const auto second = altWithNewFp(i1, fp, v.numBuckets);
         if (tryInsertAtBucket(v, second.bucket, second.fp)) return true;
         const bool first = (fp & 1) == 0;

@@ -1,0 +1,124 @@
+// Error-returning launch layer shared by the C++ template API
+// (CuckooFilter.hpp) and the C ABI: every operation is asynchronous on the
+// given stream and returns the first HIP error it meets.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+
+#include "binned.hpp"
+#include "kernels.hpp"
+#include "radix.hpp"
+
+namespace cuckoo {
+
+/// Packed tag of key i as computePackedTagsKernel computes it (CuckooFilter.cuh:1185-1207).
+template <typename Config>
+struct PackedTagSource {
+    using P = PackedTagOf<Config>;
+    const typename Config::KeyType* keys;
+    size_t numBuckets;
+    __device__ __forceinline__ P operator()(size_t i) const {
+        const auto c = Config::AltBucketPolicy::getCandidateBuckets(keys[i], numBuckets);
+        return (static_cast<P>(c.i1) << Config::bitsPerTag) | static_cast<P>(c.fp);
+    }
+};
+
+template <typename Config>
+struct Engine {
+    using Key = typename Config::KeyType;
+    using View = DeviceView<Config>;
+    using Packed = PackedTagOf<Config>;
+    using Binned = BinnedInsert<Config>;
+
+    static bool useBinned(size_t n, size_t numBuckets) {
+        if (!Binned::applicable(numBuckets) || n == 0) return false;
+        switch (detail::insertPathFromEnv()) {
+            case detail::InsertPath::Direct:
+                return false;
+            case detail::InsertPath::Binned:
+                return true;
+            default:
+                return n * detail::kBinnedMinBucketsPerKey >= numBuckets;
+        }
+    }
+
+    /// insertMany (CuckooFilter.cuh:457-464): the bucket-binned path for
+    /// batches dense enough to pay for staging table slices, one thread per
+    /// key with device-scope CAS otherwise.
+    static hipError_t insert(const View& v, const Key* keys, size_t n, uint32_t block, hipStream_t s,
+                             Workspace& ws) {
+        if (n == 0) return hipSuccess;
+        if (useBinned(n, v.numBuckets)) return Binned::template run<false>(v, keys, n, block, s, ws);
+        auto k = insertKernel<Config>;
+        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, n, v);
+        return hipGetLastError();
+    }
+
+    static hipError_t contains(const View& v, const Key* keys, size_t n, bool* out, uint32_t block,
+                               hipStream_t s) {
+        if (n == 0) return hipSuccess;
+        auto k = containsKernel<Config>;
+        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, out, n, v);
+        return hipGetLastError();
+    }
+
+    static hipError_t remove(const View& v, const Key* keys, size_t n, bool* out, uint32_t block,
+                             hipStream_t s) {
+        if (n == 0) return hipSuccess;
+        auto k = deleteKernel<Config>;
+        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, out, n, v);
+        return hipGetLastError();
+    }
+
+    static hipError_t packedTags(const Key* keys, size_t n, size_t nb, Packed* out, uint32_t block,
+                                 hipStream_t s) {
+        if (n == 0) return hipSuccess;
+        auto k = computePackedTagsKernel<Config>;
+        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, out, n, nb);
+        return hipGetLastError();
+    }
+
+    static hipError_t candidates(const Key* keys, size_t n, size_t nb, uint64_t* i1, uint64_t* i2,
+                                 uint32_t* fp, uint32_t block, hipStream_t s) {
+        if (n == 0) return hipSuccess;
+        auto k = candidatesKernel<Config>;
+        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, n, nb, i1, i2, fp);
+        return hipGetLastError();
+    }
+
+    /// insertManySorted (CuckooFilter.cuh:475-508): keys are radix-partitioned
+    /// by primary bucket (as packed into PackedTagType) and inserted slice by
+    /// slice through LDS.  Tables beyond 2^32 buckets take the packed-tag
+    /// sort + insertKernelSorted route instead.
+    static hipError_t insertSorted(const View& v, const Key* keys, size_t n, uint32_t block, hipStream_t s,
+                                   Workspace& ws) {
+        if (n == 0) return hipSuccess;
+        if (Binned::applicable(v.numBuckets)) return Binned::template run<true>(v, keys, n, block, s, ws);
+        constexpr size_t kChunk = size_t(1) << 31;
+        for (size_t off = 0; off < n; off += kChunk) {
+            const size_t m = std::min(kChunk, n - off);
+            const size_t vals = ((m * sizeof(Packed) + 255) / 256) * 256;
+            hipError_t e = ws.reserve(2 * vals + radix::scratchBytes(m));
+            if (e != hipSuccess) return e;
+            Packed* a = static_cast<Packed*>(ws.ptr);
+            Packed* b = reinterpret_cast<Packed*>(static_cast<char*>(ws.ptr) + vals);
+            void* scratch = static_cast<char*>(ws.ptr) + 2 * vals;
+            int hi = Config::bitsPerTag + detail::bitsFor(v.numBuckets);
+            hi = std::min<int>(hi, static_cast<int>(sizeof(Packed) * 8));
+            Packed* sorted = a;
+            e = radix::sortBits<Packed>(PackedTagSource<Config>{keys + off, v.numBuckets}, m, Config::bitsPerTag, hi,
+                                        a, b, scratch, s, &sorted);
+            if (e != hipSuccess) return e;
+            auto k = insertKernelSorted<Config>;
+            k<<<detail::gridFor(k, m, block), block, 0, s>>>(nullptr, sorted, m, v);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+};
+
+}  // namespace cuckoo

@@ -220,86 +220,4 @@ inline int bitsFor(size_t nb) {  // ceil(log2(nb))
 
 }  // namespace detail
 
-/// Error-returning launches for one configuration.  Every call is
-/// asynchronous on `stream`; callers synchronise.
-template <typename Config>
-struct Engine {
-    using Key = typename Config::KeyType;
-    using View = DeviceView<Config>;
-    using Packed = PackedTagOf<Config>;
-
-    static hipError_t insert(const View& v, const Key* keys, size_t n, uint32_t block, hipStream_t s) {
-        if (n == 0) return hipSuccess;
-        auto k = insertKernel<Config>;
-        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, n, v);
-        return hipGetLastError();
-    }
-
-    static hipError_t contains(const View& v, const Key* keys, size_t n, bool* out, uint32_t block,
-                               hipStream_t s) {
-        if (n == 0) return hipSuccess;
-        auto k = containsKernel<Config>;
-        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, out, n, v);
-        return hipGetLastError();
-    }
-
-    static hipError_t remove(const View& v, const Key* keys, size_t n, bool* out, uint32_t block,
-                             hipStream_t s) {
-        if (n == 0) return hipSuccess;
-        auto k = deleteKernel<Config>;
-        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, out, n, v);
-        return hipGetLastError();
-    }
-
-    static hipError_t packedTags(const Key* keys, size_t n, size_t nb, Packed* out, uint32_t block,
-                                 hipStream_t s) {
-        if (n == 0) return hipSuccess;
-        auto k = computePackedTagsKernel<Config>;
-        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, out, n, nb);
-        return hipGetLastError();
-    }
-
-    static hipError_t candidates(const Key* keys, size_t n, size_t nb, uint64_t* i1, uint64_t* i2,
-                                 uint32_t* fp, uint32_t block, hipStream_t s) {
-        if (n == 0) return hipSuccess;
-        auto k = candidatesKernel<Config>;
-        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, n, nb, i1, i2, fp);
-        return hipGetLastError();
-    }
-
-    /// Bytes of workspace insertSorted needs for a chunk of n keys.
-    static size_t sortedWorkspaceBytes(size_t n) {
-        const size_t vals = ((n * sizeof(Packed) + 255) / 256) * 256;
-        return 2 * vals + radix::scratchBytes(n);
-    }
-
-    /// insertManySorted (CuckooFilter.cuh:475-508): packed tags, ordered by
-    /// primary bucket, then the sorted insert kernel.  Chunks of at most 2^31
-    /// keys (radix counters are 32-bit).
-    static hipError_t insertSorted(const View& v, const Key* keys, size_t n, uint32_t block, hipStream_t s,
-                                   Workspace& ws) {
-        constexpr size_t kChunk = size_t(1) << 31;
-        for (size_t off = 0; off < n; off += kChunk) {
-            const size_t m = std::min(kChunk, n - off);
-            hipError_t e = ws.reserve(sortedWorkspaceBytes(m));
-            if (e != hipSuccess) return e;
-            const size_t vals = ((m * sizeof(Packed) + 255) / 256) * 256;
-            Packed* a = static_cast<Packed*>(ws.ptr);
-            Packed* b = reinterpret_cast<Packed*>(static_cast<char*>(ws.ptr) + vals);
-            void* scratch = static_cast<char*>(ws.ptr) + 2 * vals;
-            if ((e = packedTags(keys + off, m, v.numBuckets, a, block, s)) != hipSuccess) return e;
-            int hi = Config::bitsPerTag + detail::bitsFor(v.numBuckets);
-            hi = std::min<int>(hi, static_cast<int>(sizeof(Packed) * 8));
-            Packed* sorted = a;
-            if ((e = radix::sortBits<Packed>(a, b, m, Config::bitsPerTag, hi, scratch, s, &sorted)) != hipSuccess) {
-                return e;
-            }
-            auto k = insertKernelSorted<Config>;
-            k<<<detail::gridFor(k, m, block), block, 0, s>>>(nullptr, sorted, m, v);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
-        }
-        return hipSuccess;
-    }
-};
-
 }  // namespace cuckoo

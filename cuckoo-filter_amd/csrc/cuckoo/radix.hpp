@@ -1,16 +1,21 @@
-// LSD radix sort over a bit range of 32/64-bit values, written for wave64.
+// LSD radix partition over a bit range of 32/64-bit values, written for wave64.
 //
-// Used by insertManySorted to order packed tags ((primary bucket << bits) |
-// fp, CuckooFilter.cuh:1185-1207) by primary bucket.  Only the bucket-index
-// bits are sorted (ceil(log2 numBuckets) bits, 8 per pass): ordering inside a
-// bucket has no effect on what a concurrent insert may produce, so the fp
-// bits that a full 64-bit sort (CuckooFilter.cuh:485-498) would also order
-// are skipped -- 3 passes instead of 8 for a 2^24-bucket table.
+// Used by the bucket-binned insert (binned.hpp) to group packed
+// (primary bucket, fingerprint) entries by the table slice their primary
+// bucket falls in.  Only the slice-index bits are sorted, 8 per pass: order
+// inside a slice has no effect on what a concurrent insert may produce, so
+// the reference's full 64-bit sort (CuckooFilter.cuh:485-498) is not needed.
 //
-// Each pass is count -> scan -> scatter.  The scatter is stable: a tile of
-// kTile values is split into 64-value rows, each wave owns a contiguous run of
-// rows, and a value's rank inside its row comes from eight ballots (one per
-// digit bit) -- no LDS atomics, no per-lane sort.
+// Each pass is count -> scan -> scatter over tiles of kTile values:
+//  * count: per-tile digit histogram in LDS;
+//  * scan: exclusive scan of the digit-major (digit, tile) count matrix;
+//  * scatter: a tile is split into 64-value rows, each wave owns a contiguous
+//    run of rows, a value's rank inside its row comes from eight ballots (one
+//    per digit bit); values are first placed digit-ordered in an LDS staging
+//    tile, then written out so that each digit's run is a contiguous,
+//    coalesced store.  The scatter is stable, as LSD needs.
+// The first pass reads its values through a source functor, so the values
+// (e.g. entries computed from keys) need not be materialised beforehand.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,14 +29,22 @@ inline constexpr int kDigitBits = 8;
 inline constexpr int kRadix = 1 << kDigitBits;
 inline constexpr int kBlock = 512;
 inline constexpr int kWaves = kBlock / 64;
-inline constexpr int kItems = 16;
-inline constexpr int kTile = kBlock * kItems;      // 8192 values per tile
+inline constexpr int kItems = 8;
+inline constexpr int kTile = kBlock * kItems;      // 4096 values per tile (staging <= 32 KiB of LDS)
 inline constexpr int kRowsPerWave = kTile / 64 / kWaves;
 inline constexpr int kScanBlock = 1024;
 inline constexpr int kScanItems = 4;
 inline constexpr int kScanChunk = kScanBlock * kScanItems;
 
 static_assert(kTile % (64 * kWaves) == 0, "tile must split evenly into wave rows");
+static_assert(kRowsPerWave == kItems, "each lane keeps one value per row in registers");
+
+/// Reads element i of an array.
+template <typename U>
+struct ArraySource {
+    const U* data;
+    __device__ __forceinline__ U operator()(size_t i) const { return data[i]; }
+};
 
 __device__ __forceinline__ uint32_t waveInclusiveScan(uint32_t v) {
     const int lane = __lane_id();
@@ -64,9 +77,9 @@ __device__ __forceinline__ uint32_t blockExclusiveScan(uint32_t v, uint32_t& tot
     return waveBase + inc - v;
 }
 
-template <typename U>
-__global__ __launch_bounds__(kBlock) void countKernel(const U* __restrict__ in, size_t n, int shift,
-                                                      uint32_t* __restrict__ counts, uint32_t numTiles) {
+template <typename U, typename Src>
+__global__ __launch_bounds__(kBlock) void countKernel(Src src, size_t n, int shift, uint32_t* __restrict__ counts,
+                                                      uint32_t numTiles) {
     __shared__ uint32_t hist[kRadix];
     for (int i = threadIdx.x; i < kRadix; i += kBlock) hist[i] = 0;
     __syncthreads();
@@ -74,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void countKernel(const U* __restrict__ in, 
 #pragma unroll 4
     for (int k = 0; k < kItems; ++k) {
         const size_t idx = base + static_cast<size_t>(k) * kBlock + threadIdx.x;
-        if (idx < n) atomicAdd(&hist[(in[idx] >> shift) & (kRadix - 1)], 1u);
+        if (idx < n) atomicAdd(&hist[static_cast<uint32_t>(static_cast<U>(src(idx)) >> shift) & (kRadix - 1)], 1u);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kRadix; i += kBlock) counts[static_cast<size_t>(i) * numTiles + blockIdx.x] = hist[i];
@@ -144,11 +157,21 @@ __device__ __forceinline__ uint64_t digitPeers(uint32_t d, bool valid) {
     return peers;
 }
 
+/// Dynamic LDS bytes the scatter kernel needs for value type U.
 template <typename U>
-__global__ __launch_bounds__(kBlock) void scatterKernel(const U* __restrict__ in, U* __restrict__ out, size_t n,
-                                                        int shift, const uint32_t* __restrict__ offsets,
-                                                        uint32_t numTiles) {
+constexpr size_t scatterLdsBytes() {
+    return static_cast<size_t>(kTile) * sizeof(U);
+}
+
+template <typename U, typename Src>
+__global__ __launch_bounds__(kBlock) void scatterKernel(Src src, U* __restrict__ out, size_t n, int shift,
+                                                        const uint32_t* __restrict__ offsets, uint32_t numTiles) {
+    extern __shared__ __align__(16) unsigned char ldsRaw[];
+    U* stage = reinterpret_cast<U*>(ldsRaw);
     __shared__ uint32_t cursor[kWaves][kRadix];
+    __shared__ uint32_t digitStart[kRadix];
+    __shared__ uint32_t globalBase[kRadix];
+
     const int lane = __lane_id();
     const int wave = threadIdx.x / 64;
     for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&cursor[0][0])[i] = 0;
@@ -157,19 +180,35 @@ __global__ __launch_bounds__(kBlock) void scatterKernel(const U* __restrict__ in
     const size_t tileBase = static_cast<size_t>(blockIdx.x) * kTile;
     const size_t waveBase = tileBase + static_cast<size_t>(wave) * kRowsPerWave * 64;
     const uint64_t ltMask = (1ull << lane) - 1;
+    const size_t tileCount = n - tileBase < static_cast<size_t>(kTile) ? n - tileBase : static_cast<size_t>(kTile);
 
-    // per-wave digit counts over this wave's rows
+    // 1. count digits per wave; the row loop stays rolled (occupancy hides the
+    //    load latency) and the 8 digits of a lane are packed into one register
+    static_assert(kRowsPerWave * kDigitBits <= 64, "digits of a lane must pack into 64 bits");
+    uint64_t digits = 0;
+#pragma unroll 1
     for (int r = 0; r < kRowsPerWave; ++r) {
         const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
         const bool valid = idx < n;
-        const uint32_t d = valid ? static_cast<uint32_t>((in[idx] >> shift) & (kRadix - 1)) : 0;
+        const uint32_t d = valid ? static_cast<uint32_t>(static_cast<U>(src(idx)) >> shift) & (kRadix - 1) : 0u;
+        digits |= static_cast<uint64_t>(d) << (r * kDigitBits);
         const uint64_t peers = digitPeers(d, valid);
         if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
     }
     __syncthreads();
-    // turn counts into output cursors: global offset of (digit, tile) + earlier waves
-    for (int d = threadIdx.x; d < kRadix; d += kBlock) {
-        uint32_t run = offsets[static_cast<size_t>(d) * numTiles + blockIdx.x];
+    // 2. digit start inside the tile, each wave's offset inside the digit, and the global base
+    uint32_t tileTotal = 0;
+    if (threadIdx.x < kRadix) {
+        const int d = threadIdx.x;
+        for (int w = 0; w < kWaves; ++w) tileTotal += cursor[w][d];
+    }
+    uint32_t total;
+    const uint32_t start = blockExclusiveScan<kBlock>(threadIdx.x < kRadix ? tileTotal : 0u, total);
+    if (threadIdx.x < kRadix) {
+        const int d = threadIdx.x;
+        digitStart[d] = start;
+        globalBase[d] = offsets[static_cast<size_t>(d) * numTiles + blockIdx.x];
+        uint32_t run = start;
         for (int w = 0; w < kWaves; ++w) {
             const uint32_t c = cursor[w][d];
             cursor[w][d] = run;
@@ -177,20 +216,28 @@ __global__ __launch_bounds__(kBlock) void scatterKernel(const U* __restrict__ in
         }
     }
     __syncthreads();
+    // 3. stage values digit-ordered (stable) in LDS
+#pragma unroll 1
     for (int r = 0; r < kRowsPerWave; ++r) {
         const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
         const bool valid = idx < n;
-        const U val = valid ? in[idx] : U(0);
-        const uint32_t d = valid ? static_cast<uint32_t>((val >> shift) & (kRadix - 1)) : 0;
+        const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
         const uint64_t peers = digitPeers(d, valid);
         const uint32_t pos = cursor[wave][d] + static_cast<uint32_t>(__popcll(peers & ltMask));
-        if (valid) out[pos] = val;
+        if (valid) stage[pos] = static_cast<U>(src(idx));
         if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
+    }
+    __syncthreads();
+    // 4. contiguous runs per digit to their global place
+    for (uint32_t j = threadIdx.x; j < tileCount; j += kBlock) {
+        const U v = stage[j];
+        const uint32_t d = static_cast<uint32_t>(v >> shift) & (kRadix - 1);
+        out[globalBase[d] + (j - digitStart[d])] = v;
     }
 }
 
-/// Scratch needed to sort n values: 2 value buffers are the caller's; this is
-/// the count/scan space in bytes.
+/// Scratch needed to partition n values: the two value buffers are the
+/// caller's; this is the count/scan space in bytes.
 inline size_t scratchBytes(size_t n) {
     const size_t tiles = (n + kTile - 1) / kTile;
     const size_t m = tiles * kRadix;
@@ -198,31 +245,45 @@ inline size_t scratchBytes(size_t n) {
     return (m + parts) * sizeof(uint32_t) + 256;
 }
 
-/// Sorts n values by bits [lo, hi).  Ping-pongs between a and b; returns the
-/// buffer that holds the sorted result.  n must be below 2^32.
-template <typename U>
-inline hipError_t sortBits(U* a, U* b, size_t n, int lo, int hi, void* scratch, hipStream_t stream,
+/// Sorts n values by bits [lo, hi), 8 bits per pass.  The first pass reads
+/// through `first` (which may compute the values); later passes ping-pong
+/// between a and b.  *result receives the buffer holding the sorted values.
+/// n must be below 2^32 and hi > lo.
+template <typename U, typename Src>
+inline hipError_t sortBits(Src first, size_t n, int lo, int hi, U* a, U* b, void* scratch, hipStream_t stream,
                            U** result) {
     *result = a;
-    if (n <= 1 || hi <= lo) return hipSuccess;
+    if (n == 0 || hi <= lo) return hipSuccess;
     const uint32_t tiles = static_cast<uint32_t>((n + kTile - 1) / kTile);
     const size_t m = static_cast<size_t>(tiles) * kRadix;
     const uint32_t parts = static_cast<uint32_t>((m + kScanChunk - 1) / kScanChunk);
     uint32_t* counts = static_cast<uint32_t*>(scratch);
     uint32_t* partial = counts + m;
-    U* src = a;
-    U* dst = b;
+    U* dst = a;
+    U* other = b;
+    bool firstPass = true;
     for (int shift = lo; shift < hi; shift += kDigitBits) {
-        countKernel<U><<<tiles, kBlock, 0, stream>>>(src, n, shift, counts, tiles);
+        if (firstPass) {
+            countKernel<U, Src><<<tiles, kBlock, 0, stream>>>(first, n, shift, counts, tiles);
+        } else {
+            countKernel<U, ArraySource<U>><<<tiles, kBlock, 0, stream>>>(ArraySource<U>{other}, n, shift, counts,
+                                                                          tiles);
+        }
         scanReduceKernel<uint32_t><<<parts, kScanBlock, 0, stream>>>(counts, m, partial);
         scanPartialsKernel<uint32_t><<<1, kScanBlock, 0, stream>>>(partial, parts);
         scanDownKernel<uint32_t><<<parts, kScanBlock, 0, stream>>>(counts, m, partial);
-        scatterKernel<U><<<tiles, kBlock, 0, stream>>>(src, dst, n, shift, counts, tiles);
-        U* t = src;
-        src = dst;
-        dst = t;
+        if (firstPass) {
+            scatterKernel<U, Src><<<tiles, kBlock, scatterLdsBytes<U>(), stream>>>(first, dst, n, shift, counts, tiles);
+        } else {
+            scatterKernel<U, ArraySource<U>><<<tiles, kBlock, scatterLdsBytes<U>(), stream>>>(
+                ArraySource<U>{other}, dst, n, shift, counts, tiles);
+        }
+        firstPass = false;
+        *result = dst;
+        U* t = dst;
+        dst = other;
+        other = t;
     }
-    *result = src;
     return hipGetLastError();
 }
 

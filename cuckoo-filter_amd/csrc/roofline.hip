@@ -2,7 +2,7 @@
 // MI355X (built into libcuckoo_roofline.so; no part of the filter library):
 //
 //  * rl_gather: per key, read `gathers` uniformly random, aligned blocks of
-//    `gather_bytes` (32 or 64) from a table, write one byte.  The same key
+//    `gather_bytes` (32, 64 or 128) from a table, write one byte.  The same key
 //    stream in and byte out as containsMany, with the hashing reduced to one
 //    multiply so the memory system is what is measured.  `dependent` makes
 //    each further block's address depend on the data of the previous one
@@ -26,15 +26,18 @@ __device__ __forceinline__ uint64_t mix(uint64_t z) {
 
 using V4 = unsigned int __attribute__((ext_vector_type(4)));
 
-template <int GB>
+template <int GB, bool NT>
 __device__ __forceinline__ unsigned int readBlock(const V4* p) {
-    V4 acc = p[0];
+    V4 v[GB / 16];
 #pragma unroll
-    for (int i = 1; i < GB / 16; ++i) acc ^= p[i];
+    for (int i = 0; i < GB / 16; ++i) v[i] = NT ? __builtin_nontemporal_load(p + i) : p[i];
+    V4 acc = v[0];
+#pragma unroll
+    for (int i = 1; i < GB / 16; ++i) acc ^= v[i];
     return acc.x ^ acc.y ^ acc.z ^ acc.w;
 }
 
-template <int GB>
+template <int GB, bool NT>
 __global__ __launch_bounds__(1024) void gatherKernel(const V4* __restrict__ table, uint64_t blockMask,
                                                      const uint64_t* __restrict__ keys, size_t n,
                                                      uint8_t* __restrict__ out, int gathers, int dependent) {
@@ -42,10 +45,10 @@ __global__ __launch_bounds__(1024) void gatherKernel(const V4* __restrict__ tabl
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint64_t k = keys[i];
         uint64_t h = mix(k);
-        unsigned int acc = readBlock<GB>(table + (h & blockMask) * (GB / 16));
+        unsigned int acc = readBlock<GB, NT>(table + (h & blockMask) * (GB / 16));
         for (int g = 1; g < gathers; ++g) {
             h = mix(h + g + (dependent ? acc : 0u));
-            acc ^= readBlock<GB>(table + (h & blockMask) * (GB / 16));
+            acc ^= readBlock<GB, NT>(table + (h & blockMask) * (GB / 16));
         }
         out[i] = acc == 0x5A5A5A5Au;
     }
@@ -86,20 +89,27 @@ bool pow2(uint64_t v) { return v && !(v & (v - 1)); }
 extern "C" {
 
 // Returns 0 on success, -1 on a bad argument, else the HIP error code.
+// gather_bytes is 32, 64 or 128; nontemporal selects loads with the NT bit.
 int rl_gather(const void* table, size_t table_bytes, int gather_bytes, const uint64_t* keys, size_t n,
-              uint8_t* out, int gathers, int dependent, int block, void* stream) {
-    if (!table || !keys || !out || (gather_bytes != 32 && gather_bytes != 64) || gathers < 1 || block < 64 ||
-        block > 1024 || table_bytes % gather_bytes || !pow2(table_bytes / gather_bytes))
+              uint8_t* out, int gathers, int dependent, int nontemporal, int block, void* stream) {
+    if (!table || !keys || !out || (gather_bytes != 32 && gather_bytes != 64 && gather_bytes != 128) ||
+        gathers < 1 || block < 64 || block > 1024 || table_bytes % gather_bytes ||
+        !pow2(table_bytes / gather_bytes))
         return -1;
     const uint64_t mask = table_bytes / gather_bytes - 1;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    const V4* t = static_cast<const V4*>(table);
+#define RL_GATHER(GB, NT)                                                                                   \
+    gatherKernel<GB, NT><<<gridFor((const void*)gatherKernel<GB, NT>, n, block), block, 0, s>>>(t, mask, keys, \
+                                                                                              n, out, gathers, dependent)
     if (gather_bytes == 32) {
-        gatherKernel<32><<<gridFor((const void*)gatherKernel<32>, n, block), block, 0, s>>>(
-            static_cast<const V4*>(table), mask, keys, n, out, gathers, dependent);
+        if (nontemporal) RL_GATHER(32, true); else RL_GATHER(32, false);
+    } else if (gather_bytes == 64) {
+        if (nontemporal) RL_GATHER(64, true); else RL_GATHER(64, false);
     } else {
-        gatherKernel<64><<<gridFor((const void*)gatherKernel<64>, n, block), block, 0, s>>>(
-            static_cast<const V4*>(table), mask, keys, n, out, gathers, dependent);
+        if (nontemporal) RL_GATHER(128, true); else RL_GATHER(128, false);
     }
+#undef RL_GATHER
     return static_cast<int>(hipGetLastError());
 }
 

@@ -4,6 +4,8 @@ Runs the kernels of libcuckoo_roofline.so with HIP events:
   gather32 / gather64 : uniformly random aligned 32-/64-byte block reads,
                         8-byte key in and 1-byte result out per key, over a
                         table of the filter's size (512 MiB) and 4x that;
+  gather128           : random aligned 128-byte reads (one full L2 line);
+  gather32nt          : gather32 with non-temporal loads;
   gather32x2 dep      : two dependent 32-byte reads per key (lookup miss shape);
   cas8                : one random 64-bit CAS per key;
   copy                : streaming copy (HBM bandwidth reference).
@@ -27,7 +29,7 @@ def _lib():
         raise ImportError(f"{RLIB_PATH} is not built; run python cuckoo-filter_amd/build.py")
     L = C.CDLL(RLIB_PATH)
     P, sz, i = C.c_void_p, C.c_size_t, C.c_int
-    L.rl_gather.argtypes = [P, sz, i, P, sz, P, i, i, i, P]
+    L.rl_gather.argtypes = [P, sz, i, P, sz, P, i, i, i, i, P]
     L.rl_cas.argtypes = [P, sz, P, sz, i, P]
     L.rl_copy.argtypes = [P, P, sz, i, P]
     for f in (L.rl_gather, L.rl_cas, L.rl_copy):
@@ -60,10 +62,10 @@ def measure(table_bytes=512 << 20, n=1 << 26, reps=5, block=256, big_factor=4) -
     out = torch.empty(n, dtype=torch.uint8, device=dev)
     res = {"keys": n, "block": block}
 
-    def gather(tab, tb, gb, g, dep):
+    def gather(tab, tb, gb, g, dep, nt=0):
         def run():
             rc = L.rl_gather(C.c_void_p(tab.data_ptr()), tb, gb, C.c_void_p(keys.data_ptr()), n,
-                             C.c_void_p(out.data_ptr()), g, dep, block, s)
+                             C.c_void_p(out.data_ptr()), g, dep, nt, block, s)
             assert rc == 0, rc
         t = _time(torch, run, reps)
         return (8 + 1 + g * gb) * n / t / 1e9
@@ -74,6 +76,8 @@ def measure(table_bytes=512 << 20, n=1 << 26, reps=5, block=256, big_factor=4) -
         tag = f"{tb >> 20}MiB"
         res[f"gather32_{tag}_gbs"] = round(gather(tab, tb, 32, 1, 0), 1)
         res[f"gather64_{tag}_gbs"] = round(gather(tab, tb, 64, 1, 0), 1)
+        res[f"gather128_{tag}_gbs"] = round(gather(tab, tb, 128, 1, 0), 1)
+        res[f"gather32nt_{tag}_gbs"] = round(gather(tab, tb, 32, 1, 0, 1), 1)
         res[f"gather32x2dep_{tag}_gbs"] = round(gather(tab, tb, 32, 2, 1), 1)
 
         def cas():

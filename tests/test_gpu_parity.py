@@ -144,8 +144,12 @@ SEQ_CFGS = [
 ]
 
 
+@pytest.mark.parametrize("path", ["direct", "binned"])
 @pytest.mark.parametrize("kw", SEQ_CFGS, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
-def test_single_key_calls_bit_exact(cf, oracle_mod, kw):
+def test_single_key_calls_bit_exact(cf, oracle_mod, kw, path, monkeypatch):
+    """insertMany's two device paths (one thread per key with device CAS; the
+    bucket-binned LDS path) both reproduce the sequential oracle call by call."""
+    monkeypatch.setenv("CUCKOO_INSERT_PATH", path)
     O = oracle_mod
     c = cf.make_config(**kw)
     oc = _oracle_cfg(O, c)
@@ -216,9 +220,12 @@ BATCH_CFGS = [
 ]
 
 
-@pytest.mark.parametrize("sorted_insert", [False, True], ids=["plain", "sorted"])
+@pytest.mark.parametrize("mode", ["direct", "binned", "sorted"])
 @pytest.mark.parametrize("kw", BATCH_CFGS, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
-def test_batch_insert_invariants_and_binomial_parity(cf, oracle_mod, kw, sorted_insert):
+def test_batch_insert_invariants_and_binomial_parity(cf, oracle_mod, kw, mode, monkeypatch):
+    sorted_insert = mode == "sorted"
+    if not sorted_insert:
+        monkeypatch.setenv("CUCKOO_INSERT_PATH", mode)
     O = oracle_mod
     c = cf.make_config(**kw)
     oc = _oracle_cfg(O, c)
@@ -259,7 +266,9 @@ def test_batch_insert_invariants_and_binomial_parity(cf, oracle_mod, kw, sorted_
     f.close()
 
 
-def test_eviction_histogram_accounts_for_every_insert(cf, oracle_mod):
+@pytest.mark.parametrize("path", ["direct", "binned"])
+def test_eviction_histogram_accounts_for_every_insert(cf, oracle_mod, path, monkeypatch):
+    monkeypatch.setenv("CUCKOO_INSERT_PATH", path)
     c = cf.make_config(eviction="bfs", max_evictions=500)
     f = cf.CuckooFilter(1 << 20, config=c)
     f.setEvictionHistogram(True)

@@ -12,8 +12,11 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("sorted_insert", [False, True], ids=["insertMany", "insertManySorted"])
-def test_hbm_scale_fill_lookup_delete(cf, sorted_insert):
+@pytest.mark.parametrize("mode", ["direct", "binned", "sorted"])
+def test_hbm_scale_fill_lookup_delete(cf, mode, monkeypatch):
+    sorted_insert = mode == "sorted"
+    if not sorted_insert:
+        monkeypatch.setenv("CUCKOO_INSERT_PATH", mode)
     import torch
     import workloads
 
