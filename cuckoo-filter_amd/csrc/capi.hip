@@ -7,12 +7,17 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <memory>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "capi_table.hpp"
+
+extern "C" int cfk_encode_kmers_device(const void* d_seq, size_t len, uint32_t k, int canonical, uint64_t* d_out,
+                                       void* stream);
 
 namespace {
 
@@ -443,6 +448,133 @@ int cf_compute_packed_tags(cf_filter* f, const void* d_keys, size_t n, void* d_p
     hipError_t e = f->ops->packedTags(d_keys, n, f->numBuckets, d_packed, f->cfg.block_size, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e == hipSuccess ? CF_OK : hipFail(e, "compute packed tags");
+}
+
+static_assert(sizeof(cf_table_header) == 80, "table file header layout");
+
+int cf_save_table(cf_filter* f, const char* path) {
+    int r = checkFilter(f);
+    if (r != CF_OK) return r;
+    if (path == nullptr) return fail(CF_ERR_INVALID, "path is NULL");
+    cf_table_header h{};
+    std::memcpy(h.magic, "CFAMDTB1", 8);
+    h.header_bytes = sizeof(cf_table_header);
+    h.config = f->cfg;
+    h.num_buckets = f->numBuckets;
+    DeviceGuard g(f->device);
+    unsigned long long counters[2] = {0, 0};
+    hipError_t e = hipMemcpy(counters, f->counters, sizeof(counters), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hipFail(e, "reading counters");
+    h.occupied = counters[0];
+    h.evictions = counters[1];
+    h.image_bytes = f->tableBytes();
+    std::vector<unsigned char> img;
+    try {
+        img.resize(f->tableBytes());
+    } catch (...) {
+        return fail(CF_ERR_ALLOC, "host allocation of %zu bytes failed", f->tableBytes());
+    }
+    r = cf_export_table(f, img.data(), img.size());
+    if (r != CF_OK) return r;
+    std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(path, "wb"), &std::fclose);
+    if (!fp) return fail(CF_ERR_INVALID, "cannot open %s for writing", path);
+    if (std::fwrite(&h, sizeof(h), 1, fp.get()) != 1 || std::fwrite(img.data(), 1, img.size(), fp.get()) != img.size())
+        return fail(CF_ERR_INVALID, "short write to %s", path);
+    return CF_OK;
+}
+
+int cf_load_table(const char* path, cf_filter** out) {
+    if (out == nullptr || path == nullptr) return fail(CF_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(path, "rb"), &std::fclose);
+    if (!fp) return fail(CF_ERR_INVALID, "cannot open %s", path);
+    cf_table_header h{};
+    if (std::fread(&h, sizeof(h), 1, fp.get()) != 1 || std::memcmp(h.magic, "CFAMDTB1", 8) != 0 ||
+        h.header_bytes != sizeof(cf_table_header))
+        return fail(CF_ERR_INVALID, "%s is not a cuckoo filter table file", path);
+    int r = validate(&h.config);
+    if (r != CF_OK) return r;
+    const cfabi::Ops* ops = findOps(&h.config);
+    if (ops == nullptr) return fail(CF_ERR_UNSUPPORTED, "configuration not built into this library");
+    // capacity that reproduces num_buckets under the configuration's policy
+    const size_t capacity = static_cast<size_t>(h.num_buckets) * h.config.bucket_size;
+    if (ops->numBuckets(capacity) != h.num_buckets)
+        return fail(CF_ERR_INVALID, "bucket count %llu is not one this policy produces",
+                    static_cast<unsigned long long>(h.num_buckets));
+    if (h.image_bytes != h.num_buckets * ops->bucketBytes)
+        return fail(CF_ERR_INVALID, "%s: image size does not match its header", path);
+    cf_filter* f = nullptr;
+    r = cf_create(&h.config, capacity, &f);
+    if (r != CF_OK) return r;
+    std::vector<unsigned char> img;
+    try {
+        img.resize(f->tableBytes());
+    } catch (...) {
+        cf_destroy(f);
+        return fail(CF_ERR_ALLOC, "host allocation failed");
+    }
+    if (std::fread(img.data(), 1, img.size(), fp.get()) != img.size()) {
+        cf_destroy(f);
+        return fail(CF_ERR_INVALID, "%s: table image is truncated", path);
+    }
+    r = cf_import_table(f, img.data(), img.size(), static_cast<size_t>(h.occupied));
+    if (r == CF_OK) {
+        DeviceGuard g(f->device);
+        const unsigned long long ev = h.evictions;
+        const hipError_t e = hipMemcpy(f->counters + 1, &ev, sizeof(ev), hipMemcpyHostToDevice);
+        if (e != hipSuccess) r = hipFail(e, "restoring the eviction counter");
+    }
+    if (r != CF_OK) {
+        cf_destroy(f);
+        return r;
+    }
+    *out = f;
+    return CF_OK;
+}
+
+int cf_kmer_file_write(const char* path, const uint64_t* h_keys, size_t n) {
+    if (path == nullptr || (n != 0 && h_keys == nullptr)) return fail(CF_ERR_INVALID, "NULL argument");
+    std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(path, "wb"), &std::fclose);
+    if (!fp) return fail(CF_ERR_INVALID, "cannot open %s for writing", path);
+    const uint64_t count = n;
+    if (std::fwrite(&count, sizeof(count), 1, fp.get()) != 1 ||
+        (n != 0 && std::fwrite(h_keys, sizeof(uint64_t), n, fp.get()) != n))
+        return fail(CF_ERR_INVALID, "short write to %s", path);
+    return CF_OK;
+}
+
+int cf_kmer_file_count(const char* path, size_t* n) {
+    if (path == nullptr || n == nullptr) return fail(CF_ERR_INVALID, "NULL argument");
+    std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(path, "rb"), &std::fclose);
+    if (!fp) return fail(CF_ERR_INVALID, "cannot open %s", path);
+    uint64_t count = 0;
+    if (std::fread(&count, sizeof(count), 1, fp.get()) != 1) return fail(CF_ERR_INVALID, "%s: no k-mer count", path);
+    *n = static_cast<size_t>(count);
+    return CF_OK;
+}
+
+int cf_kmer_file_read(const char* path, uint64_t* h_keys, size_t n) {
+    if (path == nullptr || (n != 0 && h_keys == nullptr)) return fail(CF_ERR_INVALID, "NULL argument");
+    std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(path, "rb"), &std::fclose);
+    if (!fp) return fail(CF_ERR_INVALID, "cannot open %s", path);
+    uint64_t count = 0;
+    if (std::fread(&count, sizeof(count), 1, fp.get()) != 1) return fail(CF_ERR_INVALID, "%s: no k-mer count", path);
+    if (count != n) return fail(CF_ERR_INVALID, "%s holds %llu k-mers, not %zu", path,
+                                static_cast<unsigned long long>(count), n);
+    if (n != 0 && std::fread(h_keys, sizeof(uint64_t), n, fp.get()) != n)
+        return fail(CF_ERR_INVALID, "%s: k-mer list is truncated", path);
+    return CF_OK;
+}
+
+int cf_encode_kmers(const void* d_seq, size_t len, uint32_t k, int canonical, uint64_t* d_out, void* stream) {
+    if (d_seq == nullptr || d_out == nullptr) return fail(CF_ERR_INVALID, "NULL argument");
+    if (k == 0 || k > 32) return fail(CF_ERR_INVALID, "k must be in [1, 32]");
+    if (len < k) return fail(CF_ERR_INVALID, "sequence shorter than k");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int rc = cfk_encode_kmers_device(d_seq, len, k, canonical, d_out, stream);
+    if (rc != 0) return hipFail(static_cast<hipError_t>(rc), "encode k-mers");
+    const hipError_t e = hipStreamSynchronize(s);
+    return e == hipSuccess ? CF_OK : hipFail(e, "encode k-mers");
 }
 
 }  // extern "C"

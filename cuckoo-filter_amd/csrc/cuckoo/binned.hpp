@@ -71,13 +71,13 @@ __global__ __launch_bounds__(1024) void packEntriesKernel(Src src, size_t n, uin
 }
 
 /// start[s] = first entry whose slice is >= s (entries sorted by slice), for
-/// s in [0, numSlices]; also zeroes the overflow counter.
+/// s in [0, numSlices]; also zeroes the three list counters of the insert.
 template <typename T = uint64_t>
 __global__ __launch_bounds__(256) void sliceStartKernel(const T* __restrict__ entries, uint32_t n, int sliceShift,
                                                         uint32_t numSlices, uint32_t* __restrict__ start,
                                                         unsigned long long* overflowCount) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0) *overflowCount = 0;
+    if (s < 3) overflowCount[s] = 0;  // overflow, eviction-start and DFS list counters
     if (s > numSlices) return;
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
@@ -141,8 +141,10 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const uint64_t* __r
     uint32_t ok = 0;
     const int lane = __lane_id();
     const uint64_t ltMask = (1ull << lane) - 1;
+    uint64_t nextEntry = lo + threadIdx.x < hi ? entries[lo + threadIdx.x] : 0;
     for (uint32_t e = lo + threadIdx.x; e < hi; e += kBinBlock) {
-        const uint64_t entry = entries[e];
+        const uint64_t entry = nextEntry;
+        if (e + kBinBlock < hi) nextEntry = entries[e + kBinBlock];  // one entry ahead
         const size_t local = static_cast<size_t>(entry >> 32) - b0;
         const Tag fp = static_cast<Tag>(entry);
         Word* bw = slice + local * WC;
@@ -177,21 +179,123 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const uint64_t* __r
     detail::blockCountAdd(ok, v.occupied, false);
 }
 
-/// Step 3: the overflow entries continue from their secondary bucket.
+// Step 3 runs as three kernels over compacted lists, so that the lanes of a
+// wave do the same kind of work: the secondary-bucket attempt of every
+// overflow entry; the BFS stage of the entries whose secondary bucket was
+// full; DFS chains for the entries BFS could not place.  Each list entry is
+// (bucket << 32 | fingerprint as stored in that bucket).  Deferring a key
+// between these steps is again an interleaving the concurrent reference
+// allows.
+
+inline constexpr int kStageBlock = 1024;
+inline constexpr uint32_t kStageChunk = 2048;  // entries per workgroup round (LDS list buffer)
+
+/// Appends the values of the lanes with `emit` to the workgroup's LDS list
+/// (one LDS atomic per wave).
+__device__ __forceinline__ void waveAppend(bool emit, uint64_t value, uint64_t* buf, uint32_t* bufCount) {
+    const uint64_t mask = __ballot(emit);
+    if (mask == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(bufCount, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader, 64);
+    if (emit) buf[base + __popcll(mask & ((1ull << lane) - 1))] = value;
+}
+
+/// Secondary-bucket attempt of each overflow entry (i1 << 32 | fp); entries
+/// whose secondary bucket is full go to `next` as their eviction start.
 template <typename Config>
-__global__ __launch_bounds__(kMaxBlock) void insertOverflowKernel(const uint64_t* __restrict__ overflow,
-                                                                  const unsigned long long* __restrict__ count,
-                                                                  DeviceView<Config> v) {
+__global__ __launch_bounds__(kStageBlock) void secondaryKernel(const uint64_t* __restrict__ in,
+                                                               const unsigned long long* __restrict__ inCount,
+                                                               uint64_t* __restrict__ next,
+                                                               unsigned long long* nextCount, DeviceView<Config> v) {
     using Ops = FilterOps<Config>;
     using Tag = typename Config::TagType;
-    const size_t n = static_cast<size_t>(*count);
+    __shared__ uint64_t buf[kStageChunk];
+    __shared__ uint32_t bufCount;
+    __shared__ unsigned long long base;
+    const size_t n = static_cast<size_t>(*inCount);
+    uint32_t ok = 0;
+    for (size_t c0 = static_cast<size_t>(blockIdx.x) * kStageChunk; c0 < n;
+         c0 += static_cast<size_t>(gridDim.x) * kStageChunk) {
+        if (threadIdx.x == 0) bufCount = 0;
+        __syncthreads();
+        const size_t c1 = n - c0 < kStageChunk ? n : c0 + kStageChunk;
+        for (size_t i = c0 + threadIdx.x; i < c1; i += kStageBlock) {
+            const uint64_t entry = in[i];
+            const size_t i1 = static_cast<size_t>(entry >> 32);
+            const Tag fp = static_cast<Tag>(entry);
+            const auto second = Ops::altWithNewFp(i1, fp, v.numBuckets);
+            const bool placed = Ops::tryInsertAtBucket(v, second.bucket, second.fp);
+            ok += placed;
+            const auto e = Ops::evictStart(i1, fp, second);
+            waveAppend(!placed, (static_cast<uint64_t>(e.bucket) << 32) | static_cast<uint64_t>(e.fp), buf,
+                       &bufCount);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && bufCount != 0) base = atomicAdd(nextCount, static_cast<unsigned long long>(bufCount));
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < bufCount; j += kStageBlock) next[base + j] = buf[j];
+        __syncthreads();
+    }
+    detail::blockCountAdd(ok, v.occupied, false);
+}
+
+/// BFS stage of each eviction start (bucket << 32 | fp); unplaced entries go
+/// to `next` for DFS.  With the DFS policy everything goes straight on.
+template <typename Config>
+__global__ __launch_bounds__(kStageBlock) void bfsStageKernel(const uint64_t* __restrict__ in,
+                                                              const unsigned long long* __restrict__ inCount,
+                                                              uint64_t* __restrict__ next,
+                                                              unsigned long long* nextCount, DeviceView<Config> v) {
+    using Ops = FilterOps<Config>;
+    using Tag = typename Config::TagType;
+    __shared__ uint64_t buf[kStageChunk];
+    __shared__ uint32_t bufCount;
+    __shared__ unsigned long long base;
+    const size_t n = static_cast<size_t>(*inCount);
+    uint32_t ok = 0;
+    uint32_t ev = 0;
+    for (size_t c0 = static_cast<size_t>(blockIdx.x) * kStageChunk; c0 < n;
+         c0 += static_cast<size_t>(gridDim.x) * kStageChunk) {
+        if (threadIdx.x == 0) bufCount = 0;
+        __syncthreads();
+        const size_t c1 = n - c0 < kStageChunk ? n : c0 + kStageChunk;
+        for (size_t i = c0 + threadIdx.x; i < c1; i += kStageBlock) {
+            const uint64_t entry = in[i];
+            uint32_t chain = 0;
+            const bool placed = Ops::bfsCandidates(v, static_cast<Tag>(entry), static_cast<size_t>(entry >> 32), chain);
+            ok += placed;
+            ev += chain;
+            if (placed && v.chainHist != nullptr) atomicAdd(&v.chainHist[1], 1u);
+            waveAppend(!placed, entry, buf, &bufCount);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && bufCount != 0) base = atomicAdd(nextCount, static_cast<unsigned long long>(bufCount));
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < bufCount; j += kStageBlock) next[base + j] = buf[j];
+        __syncthreads();
+    }
+    detail::blockCountAdd(ok, v.occupied, false);
+    if (v.evictions != nullptr) detail::blockCountAdd(ev, v.evictions, false);
+}
+
+/// DFS chains (insertWithEvictionDFS) from each eviction start.
+template <typename Config>
+__global__ __launch_bounds__(kMaxBlock) void dfsStageKernel(const uint64_t* __restrict__ in,
+                                                            const unsigned long long* __restrict__ inCount,
+                                                            DeviceView<Config> v) {
+    using Ops = FilterOps<Config>;
+    using Tag = typename Config::TagType;
+    const size_t n = static_cast<size_t>(*inCount);
     uint32_t ok = 0;
     uint32_t ev = 0;
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint64_t entry = overflow[i];
-        uint32_t chain;
-        const bool s = Ops::insertAfterPrimary(v, static_cast<size_t>(entry >> 32), static_cast<Tag>(entry), chain);
+        const uint64_t entry = in[i];
+        uint32_t chain = 0;
+        const bool s = Ops::insertDFS(v, static_cast<Tag>(entry), static_cast<size_t>(entry >> 32), chain);
         ok += s;
         ev += chain;
         if (v.chainHist != nullptr && (chain != 0 || !s)) {
@@ -254,7 +358,7 @@ struct BinnedInsert {
 
     static size_t workspaceBytes(size_t n, size_t numBuckets) {
         return 2 * align256(n * sizeof(uint64_t)) + align256(radix::scratchBytes(n)) +
-               align256((numSlices(numBuckets) + 1) * sizeof(uint32_t)) + 256;
+               align256((numSlices(numBuckets) + 1) * sizeof(uint32_t)) + align256(3 * sizeof(unsigned long long));
     }
 
     template <bool SortedSemantics>
@@ -276,7 +380,7 @@ struct BinnedInsert {
             p += align256(radix::scratchBytes(m));
             uint32_t* start = reinterpret_cast<uint32_t*>(p);
             p += align256((slices + 1) * sizeof(uint32_t));
-            unsigned long long* overflowCount = reinterpret_cast<unsigned long long*>(p);
+            unsigned long long* counters = reinterpret_cast<unsigned long long*>(p);  // overflow, evict, dfs
 
             EntrySource<Config, SortedSemantics> src{keys + off, v.numBuckets};
             uint64_t* sorted = a;
@@ -287,14 +391,24 @@ struct BinnedInsert {
                 auto k = packEntriesKernel<decltype(src)>;
                 k<<<detail::gridFor(k, m, 256), 256, 0, s>>>(src, m, a);
             }
-            uint64_t* overflow = sorted == a ? b : a;
+            uint64_t* spare = sorted == a ? b : a;
             sliceStartKernel<uint64_t><<<static_cast<uint32_t>((slices + 1 + 255) / 256), 256, 0, s>>>(
-                sorted, static_cast<uint32_t>(m), 32 + l2s, static_cast<uint32_t>(slices), start, overflowCount);
+                sorted, static_cast<uint32_t>(m), 32 + l2s, static_cast<uint32_t>(slices), start, counters);
             const size_t lds = binLdsBytes((size_t(1) << l2s) * kBucketBytes);
+            // overflow list -> spare; eviction starts -> sorted (free by then); DFS list -> spare
             binInsertKernel<Config><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(sorted, start, l2s, v,
-                                                                                       overflow, overflowCount);
-            auto k = insertOverflowKernel<Config>;
-            k<<<detail::gridFor(k, m, block), block, 0, s>>>(overflow, overflowCount, v);
+                                                                                       spare, counters);
+            const uint32_t stageGrid = detail::gridFor(secondaryKernel<Config>, (m + kStageChunk - 1) / kStageChunk * kStageBlock,
+                                                       kStageBlock);
+            secondaryKernel<Config><<<stageGrid, kStageBlock, 0, s>>>(spare, counters, sorted, counters + 1, v);
+            if (v.evictionPolicy == static_cast<uint32_t>(EvictionPolicy::BFS)) {
+                bfsStageKernel<Config><<<stageGrid, kStageBlock, 0, s>>>(sorted, counters + 1, spare, counters + 2, v);
+                auto k = dfsStageKernel<Config>;
+                k<<<detail::gridFor(k, m, block), block, 0, s>>>(spare, counters + 2, v);
+            } else {
+                auto k = dfsStageKernel<Config>;
+                k<<<detail::gridFor(k, m, block), block, 0, s>>>(sorted, counters + 1, v);
+            }
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         return hipSuccess;

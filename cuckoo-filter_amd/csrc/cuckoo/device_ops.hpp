@@ -360,9 +360,10 @@ struct FilterOps {
         return false;
     }
 
-    /// insertWithEvictionBFS: up to min(8, bucketSize/2) candidates at slots
-    /// (fp + i*7) & (bucketSize-1), then DFS.
-    static __device__ bool insertBFS(const View& v, Tag fp, size_t startBucket, uint32_t& chain) {
+    /// The BFS stage of insertWithEvictionBFS (CuckooFilter.cuh:968-1014): up to
+    /// min(8, bucketSize/2) candidates at slots (fp + i*7) & (bucketSize-1);
+    /// false means the caller continues with DFS from the same bucket (:1017).
+    static __device__ bool bfsCandidates(const View& v, Tag fp, size_t startBucket, uint32_t& chain) {
         constexpr uint32_t numCandidates = (BS / 2) < 8 ? uint32_t(BS / 2) : 8u;
         if constexpr (numCandidates > 0) {
             Word* sb = bucketPtr(v, startBucket);
@@ -392,6 +393,12 @@ struct FilterOps {
                 }
             }
         }
+        return false;
+    }
+
+    /// insertWithEvictionBFS: the BFS stage, then DFS (CuckooFilter.cuh:967-1018).
+    static __device__ bool insertBFS(const View& v, Tag fp, size_t startBucket, uint32_t& chain) {
+        if (bfsCandidates(v, fp, startBucket, chain)) return true;
         return insertDFS(v, fp, startBucket, chain);
     }
 
@@ -410,13 +417,23 @@ struct FilterOps {
         chain = 0;
         const auto second = altWithNewFp(i1, fp, v.numBuckets);
         if (tryInsertAtBucket(v, second.bucket, second.fp)) return true;
-        const bool first = (fp & 1) == 0;
-        const Tag evictFp = first ? fp : second.fp;
-        const size_t start = first ? i1 : second.bucket;
+        const EvictStart e = evictStart(i1, fp, second);
         if (v.evictionPolicy == static_cast<uint32_t>(EvictionPolicy::BFS)) {
-            return insertBFS(v, evictFp, start, chain);
+            return insertBFS(v, e.fp, e.bucket, chain);
         }
-        return insertDFS(v, evictFp, start, chain);
+        return insertDFS(v, e.fp, e.bucket, chain);
+    }
+
+    /// Where eviction starts after both candidate buckets were full: the
+    /// bucket selected by the fingerprint's low bit, with the fingerprint as
+    /// stored there (CuckooFilter.cuh:1062-1066, 1243-1244).
+    struct EvictStart {
+        size_t bucket;
+        Tag fp;
+    };
+    static __device__ __forceinline__ EvictStart evictStart(size_t i1, Tag fp, cuckoo::AltBucket<Tag> second) {
+        const bool first = (fp & 1) == 0;
+        return {first ? i1 : second.bucket, first ? fp : second.fp};
     }
 
     static __device__ __forceinline__ bool insert(const View& v, const Key& key, uint32_t& chain) {

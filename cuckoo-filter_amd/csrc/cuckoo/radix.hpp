@@ -182,16 +182,26 @@ __global__ __launch_bounds__(kBlock) void scatterKernel(Src src, U* __restrict__
     const uint64_t ltMask = (1ull << lane) - 1;
     const size_t tileCount = n - tileBase < static_cast<size_t>(kTile) ? n - tileBase : static_cast<size_t>(kTile);
 
-    // 1. count digits per wave; the row loop stays rolled (occupancy hides the
-    //    load latency) and the 8 digits of a lane are packed into one register
+    // 1. count digits per wave.  All rows of a lane are loaded first (many
+    //    loads in flight per wave); the ballot ranking loop stays rolled, with
+    //    the lane's digits packed into one register.
     static_assert(kRowsPerWave * kDigitBits <= 64, "digits of a lane must pack into 64 bits");
+    U vals[kRowsPerWave];
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
+        vals[r] = idx < n ? static_cast<U>(src(idx)) : U(0);
+    }
     uint64_t digits = 0;
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        digits |= static_cast<uint64_t>(static_cast<uint32_t>(vals[r] >> shift) & (kRadix - 1)) << (r * kDigitBits);
+    }
 #pragma unroll 1
     for (int r = 0; r < kRowsPerWave; ++r) {
         const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
         const bool valid = idx < n;
-        const uint32_t d = valid ? static_cast<uint32_t>(static_cast<U>(src(idx)) >> shift) & (kRadix - 1) : 0u;
-        digits |= static_cast<uint64_t>(d) << (r * kDigitBits);
+        const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
         const uint64_t peers = digitPeers(d, valid);
         if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
     }
@@ -217,14 +227,14 @@ __global__ __launch_bounds__(kBlock) void scatterKernel(Src src, U* __restrict__
     }
     __syncthreads();
     // 3. stage values digit-ordered (stable) in LDS
-#pragma unroll 1
+#pragma unroll
     for (int r = 0; r < kRowsPerWave; ++r) {
         const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
         const bool valid = idx < n;
         const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
         const uint64_t peers = digitPeers(d, valid);
         const uint32_t pos = cursor[wave][d] + static_cast<uint32_t>(__popcll(peers & ltMask));
-        if (valid) stage[pos] = static_cast<U>(src(idx));
+        if (valid) stage[pos] = vals[r];
         if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
     }
     __syncthreads();

@@ -82,6 +82,12 @@ def _load():
         "cf_import_table": (i32, [P, P, sz, sz]),
         "cf_compute_candidates": (i32, [P, P, sz, P, P, P, P]),
         "cf_compute_packed_tags": (i32, [P, P, sz, P, P]),
+        "cf_save_table": (i32, [P, C.c_char_p]),
+        "cf_load_table": (i32, [C.c_char_p, C.POINTER(P)]),
+        "cf_kmer_file_write": (i32, [C.c_char_p, P, sz]),
+        "cf_kmer_file_count": (i32, [C.c_char_p, C.POINTER(sz)]),
+        "cf_kmer_file_read": (i32, [C.c_char_p, P, sz]),
+        "cf_encode_kmers": (i32, [P, sz, C.c_uint32, i32, P, P]),
     }
     del u64p, u32p
     for name, (res, args) in sig.items():
@@ -306,6 +312,25 @@ class CuckooFilter:
                                          _stream_ptr(stream)))
         return i1, i2, fp
 
+    def saveTable(self, path: str):
+        """Header + table image file (cf_save_table)."""
+        _check(lib.cf_save_table(self._h, os.fsencode(path)))
+
+    @classmethod
+    def loadTable(cls, path: str) -> "CuckooFilter":
+        """A new filter (current device) from a cf_save_table file."""
+        h = C.c_void_p()
+        _check(lib.cf_load_table(os.fsencode(path), C.byref(h)))
+        f = cls.__new__(cls)
+        f._h = h
+        cfg = cf_config()
+        # the header's configuration: re-read it from the file
+        with open(path, "rb") as fh:
+            hdr = fh.read(80)
+        C.memmove(C.addressof(cfg), hdr[16:48], 32)
+        f.config = cfg
+        return f
+
     def computePackedTags(self, keys, stream=None):
         torch = _torch()
         k = self._keys(keys)
@@ -314,6 +339,30 @@ class CuckooFilter:
         _check(lib.cf_compute_packed_tags(self._h, C.c_void_p(k.data_ptr()), k.numel(),
                                           C.c_void_p(out.data_ptr()), _stream_ptr(stream)))
         return out
+
+
+def write_kmer_file(path: str, keys: np.ndarray):
+    """k-mer key file: uint64 N then N uint64 (src/kmc_to_binary.cpp:1-7 format)."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    _check(lib.cf_kmer_file_write(os.fsencode(path), k.ctypes.data, k.size))
+
+
+def read_kmer_file(path: str) -> np.ndarray:
+    n = C.c_size_t()
+    _check(lib.cf_kmer_file_count(os.fsencode(path), C.byref(n)))
+    out = np.empty(n.value, np.uint64)
+    _check(lib.cf_kmer_file_read(os.fsencode(path), out.ctypes.data, out.size))
+    return out
+
+
+def encode_kmers(seq, k: int, canonical: bool = True, stream=None):
+    """2-bit keys of every k-mer of an ASCII DNA sequence (device uint8 tensor)."""
+    torch = _torch()
+    n = seq.numel() - k + 1
+    out = torch.empty(max(n, 0), dtype=torch.int64, device=seq.device)
+    _check(lib.cf_encode_kmers(C.c_void_p(seq.data_ptr()), seq.numel(), k, 1 if canonical else 0,
+                               C.c_void_p(out.data_ptr()), _stream_ptr(stream)))
+    return out
 
 
 def keys_to_device(keys: np.ndarray, device="cuda"):

@@ -142,6 +142,45 @@ int cf_compute_candidates(cf_filter* f, const void* d_keys, size_t n, uint64_t* 
  * PackedTagType values (uint32_t when bits_per_tag <= 8, else uint64_t). */
 int cf_compute_packed_tags(cf_filter* f, const void* d_keys, size_t n, void* d_packed, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* File formats (no reference counterpart for the table file; the k-mer key  */
+/* file is the reference's src/kmc_to_binary.cpp:1-7 format).                */
+/* ------------------------------------------------------------------------ */
+
+/* Table file: cf_table_header (80 bytes, little-endian) followed by the raw
+ * table image of num_buckets * sizeof(Bucket) bytes. */
+typedef struct cf_table_header {
+    char magic[8];          /* "CFAMDTB1" */
+    uint32_t header_bytes;  /* sizeof(cf_table_header) = 80 */
+    uint32_t reserved;      /* 0 */
+    cf_config config;       /* configuration the image was written with */
+    uint64_t num_buckets;
+    uint64_t occupied;      /* occupancy counter */
+    uint64_t evictions;     /* eviction counter */
+    uint64_t image_bytes;   /* num_buckets * sizeof(Bucket) */
+} cf_table_header;
+
+/* Writes the filter's header and table image to `path`. */
+int cf_save_table(cf_filter* f, const char* path);
+
+/* Creates a filter on the current device from a file cf_save_table wrote
+ * (same configuration, bucket count, image and counters). */
+int cf_load_table(const char* path, cf_filter** out);
+
+/* k-mer key file: uint64 N, then N uint64 keys (little-endian), as written by
+ * the reference's src/kmc_to_binary.cpp:92-123 and read by
+ * benchmark/kmer_benchmark.cu:26-47. */
+int cf_kmer_file_write(const char* path, const uint64_t* h_keys, size_t n);
+int cf_kmer_file_count(const char* path, size_t* n);
+int cf_kmer_file_read(const char* path, uint64_t* h_keys, size_t n);
+
+/* 2-bit encodes every k-mer (1 <= k <= 32) of an ASCII DNA sequence on the
+ * device: d_out[i] = key of seq[i .. i+k-1], len - k + 1 keys.  A=0 C=1 G=2
+ * T=3, first base in the highest bits (src/kmc_to_binary.cpp:17-46); with
+ * `canonical` the smaller of the k-mer and its reverse complement; a k-mer
+ * holding another character gets the key UINT64_MAX. */
+int cf_encode_kmers(const void* d_seq, size_t len, uint32_t k, int canonical, uint64_t* d_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
