@@ -95,6 +95,104 @@ def cpu_baseline(n_ins, n_look, n_del, budget_s=25.0):
     }
 
 
+def batched_section(cf, torch, workloads, dev, n_filters=256, n_batches=8192, seed=7):
+    """Batched many-filter workload: n_filters filters with capacities
+    log-uniform in 2^14..2^26 slots (16-bit tags, bucket 16, BFS), each filled
+    to a uniform-random load in 0.3..0.95, driven by n_batches batches of
+    log-uniform 32..2^20 keys, ops 40% insert / 50% lookup (50% positive) /
+    10% delete, target filter uniform.  Inserts are clipped so that no filter
+    passes 95% load.  The same window runs as plain per-batch calls and as one
+    cf_batch_execute, each from the same starting filter states."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    cfg = cf.make_config(key_bytes=8, bits_per_tag=16, bucket_size=16, eviction="bfs")
+    caps = (2 ** rng.uniform(14, 26, n_filters)).astype(np.int64)
+    base, fill_n, counts = [], [], []
+    for i in range(n_filters):
+        f = cf.CuckooFilter(int(caps[i]), config=cfg)
+        n = int(f.capacity() * rng.uniform(0.3, 0.95))
+        f.insertMany(workloads.gen_keys_torch(1000 + i, n, device=dev))
+        base.append(f)
+        fill_n.append(n)
+        counts.append(f.occupiedSlots())
+    del_cursor = [0] * n_filters
+    ins_cursor = [0] * n_filters
+    ops_spec = []
+    total_keys = 0
+    for j in range(n_batches):
+        i = int(rng.integers(n_filters))
+        m = int(2 ** rng.uniform(5, 20))
+        kind = int(rng.choice(3, p=[0.4, 0.5, 0.1]))
+        if kind == 0:
+            m = max(0, min(m, int(0.95 * base[i].capacity()) - counts[i]))
+            keys = workloads.gen_keys_torch(3000 + i, m, start=ins_cursor[i], device=dev)
+            ins_cursor[i] += m
+            counts[i] += m
+        elif kind == 1:
+            live = fill_n[i] - del_cursor[i]
+            pos = min(m // 2, live)
+            start = del_cursor[i] + int(rng.integers(0, max(1, live - pos + 1)))
+            keys = torch.cat([workloads.gen_keys_torch(1000 + i, pos, start=start, device=dev),
+                              workloads.gen_keys_torch(5000 + j, m - pos, device=dev)])
+        else:
+            m = min(m, fill_n[i] - del_cursor[i])
+            keys = workloads.gen_keys_torch(1000 + i, m, start=del_cursor[i], device=dev)
+            del_cursor[i] += m
+            counts[i] -= m
+        out = torch.empty(max(m, 1), dtype=torch.bool, device=dev) if kind else None
+        ops_spec.append((i, kind, keys, out))
+        total_keys += m
+    plain = [cf.CuckooFilter(int(caps[i]), config=cfg) for i in range(n_filters)]
+    batched = [cf.CuckooFilter(int(caps[i]), config=cfg) for i in range(n_filters)]
+
+    def restore(fs):
+        for f, b in zip(fs, base):
+            f.copyFrom(b)
+        torch.cuda.synchronize()
+
+    def run_plain():
+        occ = []
+        for i, kind, keys, out in ops_spec:
+            f = plain[i]
+            if kind == 0:
+                occ.append(f.insertMany(keys))
+            elif kind == 1:
+                f.containsMany(keys, out=out)
+                occ.append(0)
+            else:
+                occ.append(f.deleteMany(keys, out=out))
+        return occ
+
+    b = cf.Batch()
+    arr = b.make_ops([(batched[i], kind, keys, out) for i, kind, keys, out in ops_spec])
+    restore(batched)
+    b.execute(arr)  # first window: plans and captures the graph
+    restore(plain)
+    t0 = time.perf_counter()
+    occ_p = run_plain()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    restore(batched)
+    t0 = time.perf_counter()
+    occ_b = b.execute(arr)
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    same = all(int(occ_b[k]) == occ_p[k] for k in range(len(ops_spec)) if ops_spec[k][1] != 1)
+    b.close()
+    return {
+        "workload": (f"{n_filters} filters, capacities log-uniform 2^14..2^26 slots, fill U(0.3, 0.95); "
+                     f"{n_batches} batches log-uniform 32..2^20 keys, 40/50/10 insert/lookup/delete"),
+        "keys": total_keys,
+        "plain_mkeys_s": round(total_keys / t_plain / 1e6, 1),
+        "batched_mkeys_s": round(total_keys / t_batched / 1e6, 1),
+        "speedup": round(t_plain / t_batched, 2),
+        "plain_ms": round(t_plain * 1e3, 2),
+        "batched_ms": round(t_batched * 1e3, 2),
+        "occupancies_match_plain": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +201,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sorted", action="store_true", help="also time insertManySorted")
     ap.add_argument("--no-roofline", action="store_true", help="skip the measured gather/CAS roofline")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched many-filter comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the cuckoo-filter hot path has no multi-GPU leg in this build "
@@ -232,6 +331,10 @@ def main():
     del keys, pos, query, dels, out, dout
     f.close()
     torch.cuda.empty_cache()
+    if not args.no_batched:
+        log("batched many-filter window vs plain calls")
+        result["batched"] = batched_section(cf, torch, workloads, dev)
+        torch.cuda.empty_cache()
     if not args.no_roofline:
         import roofline as rl
 

@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -83,6 +85,19 @@ const cfabi::Ops* findOps(const cf_config* c) {
     return nullptr;
 }
 
+/// Workspace of the binned insert, shared by every filter on a device: calls
+/// are synchronous, so a call owns it from launch to return.  Memory held is
+/// the largest call's need, not a sum over filters.
+struct SharedWorkspace {
+    std::mutex mu;
+    std::map<int, cuckoo::Workspace> perDevice;
+};
+
+SharedWorkspace& sharedWorkspace() {
+    static SharedWorkspace* ws = new SharedWorkspace();  // lives until process exit
+    return *ws;
+}
+
 /// Makes the filter's device current for the duration of a call.
 class DeviceGuard {
    public:
@@ -111,7 +126,6 @@ struct cf_filter {
     void* buckets = nullptr;
     unsigned long long* counters = nullptr;  // [0] occupied, [1] evictions
     unsigned int* hist = nullptr;            // max_evictions + 2 bins, or nullptr
-    cuckoo::Workspace ws;
 
     size_t tableBytes() const { return numBuckets * ops->bucketBytes; }
 
@@ -232,7 +246,6 @@ int cf_create(const cf_config* cfg, size_t capacity, cf_filter** out) {
 int cf_destroy(cf_filter* f) {
     if (f == nullptr) return CF_OK;
     DeviceGuard g(f->device);
-    f->ws.release();
     hipError_t e = hipSuccess;
     if (f->buckets) e = hipFree(f->buckets);
     if (f->counters) (void)hipFree(f->counters);
@@ -247,8 +260,13 @@ int cf_insert_many(cf_filter* f, const void* d_keys, size_t n, void* stream, siz
     if (r != CF_OK) return r;
     DeviceGuard g(f->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipError_t e = f->ops->insert(f->view(), d_keys, n, f->cfg.block_size, s, f->ws);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipError_t e;
+    {
+        SharedWorkspace& sw = sharedWorkspace();
+        std::lock_guard<std::mutex> lock(sw.mu);
+        e = f->ops->insert(f->view(), d_keys, n, f->cfg.block_size, s, sw.perDevice[f->device]);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
     if (e != hipSuccess) return hipFail(e, "insertMany");
     return readOccupied(f, s, occupied);
 }
@@ -259,8 +277,13 @@ int cf_insert_many_sorted(cf_filter* f, const void* d_keys, size_t n, void* stre
     if (r != CF_OK) return r;
     DeviceGuard g(f->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipError_t e = f->ops->insertSorted(f->view(), d_keys, n, f->cfg.block_size, s, f->ws);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipError_t e;
+    {
+        SharedWorkspace& sw = sharedWorkspace();
+        std::lock_guard<std::mutex> lock(sw.mu);
+        e = f->ops->insertSorted(f->view(), d_keys, n, f->cfg.block_size, s, sw.perDevice[f->device]);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
     if (e != hipSuccess) return hipFail(e, "insertManySorted");
     return readOccupied(f, s, occupied);
 }
@@ -425,6 +448,21 @@ int cf_import_table(cf_filter* f, const void* h_src, size_t bytes, size_t occupi
     return e == hipSuccess ? CF_OK : hipFail(e, "import table");
 }
 
+int cf_copy_filter(cf_filter* dst, cf_filter* src, void* stream) {
+    int r = checkFilter(dst);
+    if (r == CF_OK) r = checkFilter(src);
+    if (r != CF_OK) return r;
+    if (dst->ops != src->ops || dst->numBuckets != src->numBuckets || dst->device != src->device)
+        return fail(CF_ERR_INVALID, "filters differ in configuration, size or device");
+    DeviceGuard g(dst->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemcpyAsync(dst->buckets, src->buckets, dst->tableBytes(), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dst->counters, src->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? CF_OK : hipFail(e, "copy filter");
+}
+
 int cf_compute_candidates(cf_filter* f, const void* d_keys, size_t n, uint64_t* d_i1, uint64_t* d_i2,
                           uint32_t* d_fp, void* stream) {
     int r = checkFilter(f);
@@ -448,6 +486,80 @@ int cf_compute_packed_tags(cf_filter* f, const void* d_keys, size_t n, void* d_p
     hipError_t e = f->ops->packedTags(d_keys, n, f->numBuckets, d_packed, f->cfg.block_size, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e == hipSuccess ? CF_OK : hipFail(e, "compute packed tags");
+}
+
+}  // extern "C"
+
+struct cf_batch {
+    std::vector<std::pair<const cfabi::Ops*, void*>> executors;  // one per configuration used
+    std::vector<cuckoo::BatchOpDesc> descs;
+};
+
+extern "C" {
+
+int cf_batch_create(cf_batch** out) {
+    if (out == nullptr) return fail(CF_ERR_INVALID, "out is NULL");
+    *out = new (std::nothrow) cf_batch();
+    return *out != nullptr ? CF_OK : fail(CF_ERR_ALLOC, "host allocation failed");
+}
+
+int cf_batch_destroy(cf_batch* b) {
+    if (b == nullptr) return CF_OK;
+    for (auto& [ops, state] : b->executors) {
+        if (state != nullptr) ops->batchRelease(state);
+    }
+    delete b;
+    return CF_OK;
+}
+
+int cf_batch_execute(cf_batch* b, const cf_batch_op* ops, size_t count, void* stream, uint64_t* h_occupied) {
+    if (b == nullptr) return fail(CF_ERR_INVALID, "batch is NULL");
+    if (count == 0) return CF_OK;
+    if (ops == nullptr) return fail(CF_ERR_INVALID, "ops is NULL");
+    const cfabi::Ops* kind = nullptr;
+    int device = -1;
+    b->descs.resize(count);
+    for (size_t i = 0; i < count; ++i) {
+        const cf_batch_op& o = ops[i];
+        int r = checkFilter(o.filter);
+        if (r != CF_OK) return r;
+        if (kind == nullptr) {
+            kind = o.filter->ops;
+            device = o.filter->device;
+        } else if (o.filter->ops != kind) {
+            return fail(CF_ERR_INVALID, "op %zu: every filter of a batch must share one configuration", i);
+        } else if (o.filter->device != device) {
+            return fail(CF_ERR_INVALID, "op %zu: every filter of a batch must live on one device", i);
+        }
+        if (o.op > CF_OP_DELETE) return fail(CF_ERR_INVALID, "op %zu: unknown operation %u", i, o.op);
+        if (o.n != 0 && o.d_keys == nullptr) return fail(CF_ERR_INVALID, "op %zu: key pointer is NULL", i);
+        if (o.op == CF_OP_LOOKUP && o.n != 0 && o.d_output == nullptr)
+            return fail(CF_ERR_INVALID, "op %zu: lookup needs an output", i);
+        cuckoo::BatchOpDesc& d = b->descs[i];
+        d.words = o.filter->buckets;
+        d.numBuckets = o.filter->numBuckets;
+        d.occupied = o.filter->counters;
+        d.evictions = o.filter->counters + 1;
+        d.chainHist = o.filter->hist;
+        d.keys = o.d_keys;
+        d.out = o.op == CF_OP_INSERT ? nullptr : o.d_output;
+        d.n = o.n;
+        d.kind = o.op;
+        d.maxEvictions = o.filter->cfg.max_evictions;
+        d.evictionPolicy = o.filter->cfg.eviction;
+        d.pad = 0;
+    }
+    void** state = nullptr;
+    for (auto& [k, st] : b->executors) {
+        if (k == kind) state = &st;
+    }
+    if (state == nullptr) {
+        b->executors.emplace_back(kind, nullptr);
+        state = &b->executors.back().second;
+    }
+    DeviceGuard g(device);
+    const hipError_t e = kind->batchExecute(state, b->descs.data(), count, static_cast<hipStream_t>(stream), h_occupied);
+    return e == hipSuccess ? CF_OK : hipFail(e, "batch execute");
 }
 
 static_assert(sizeof(cf_table_header) == 80, "table file header layout");

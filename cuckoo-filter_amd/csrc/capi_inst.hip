@@ -87,6 +87,13 @@ struct Entry {
         return E::packedTags(static_cast<const Key*>(keys), n, nb, static_cast<typename E::Packed*>(out), block, s);
     }
 
+    static hipError_t batchExecute(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t s,
+                                   uint64_t* occupiedOut) {
+        if (*state == nullptr) *state = new cuckoo::BatchExecutor<Cfg>();
+        return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->execute(ops, n, s, occupiedOut);
+    }
+    static void batchRelease(void* state) { delete static_cast<cuckoo::BatchExecutor<Cfg>*>(state); }
+
     static cfabi::Ops make() {
         cfabi::Ops o{};
         o.policy = CF_INST_POLICY;
@@ -103,6 +110,8 @@ struct Entry {
         o.remove = &remove;
         o.candidates = &candidates;
         o.packedTags = &packedTags;
+        o.batchExecute = &batchExecute;
+        o.batchRelease = &batchRelease;
         return o;
     }
 };

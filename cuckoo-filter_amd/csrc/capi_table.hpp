@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "cuckoo/batch.hpp"
 #include "cuckoo/engine.hpp"
 
 namespace cfabi {
@@ -37,6 +38,10 @@ struct Ops {
     hipError_t (*candidates)(const void* keys, size_t n, size_t nb, uint64_t* i1, uint64_t* i2, uint32_t* fp,
                              uint32_t block, hipStream_t);
     hipError_t (*packedTags)(const void* keys, size_t n, size_t nb, void* out, uint32_t block, hipStream_t);
+    // batched many-filter execution; *state is this configuration's executor (created on first use)
+    hipError_t (*batchExecute)(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t,
+                               uint64_t* occupiedOut);
+    void (*batchRelease)(void* state);
 };
 
 }  // namespace cfabi

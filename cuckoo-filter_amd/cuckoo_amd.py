@@ -31,6 +31,20 @@ class CuckooError(RuntimeError):
         self.code = code
 
 
+OP_INSERT, OP_LOOKUP, OP_DELETE = 0, 1, 2
+
+
+class cf_batch_op(C.Structure):
+    _fields_ = [
+        ("filter", C.c_void_p),
+        ("op", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("d_keys", C.c_void_p),
+        ("n", C.c_size_t),
+        ("d_output", C.c_void_p),
+    ]
+
+
 class cf_config(C.Structure):
     _fields_ = [
         ("key_bytes", C.c_uint32),
@@ -82,6 +96,10 @@ def _load():
         "cf_import_table": (i32, [P, P, sz, sz]),
         "cf_compute_candidates": (i32, [P, P, sz, P, P, P, P]),
         "cf_compute_packed_tags": (i32, [P, P, sz, P, P]),
+        "cf_copy_filter": (i32, [P, P, P]),
+        "cf_batch_create": (i32, [C.POINTER(P)]),
+        "cf_batch_destroy": (i32, [P]),
+        "cf_batch_execute": (i32, [P, P, sz, P, P]),
         "cf_save_table": (i32, [P, C.c_char_p]),
         "cf_load_table": (i32, [C.c_char_p, C.POINTER(P)]),
         "cf_kmer_file_write": (i32, [C.c_char_p, P, sz]),
@@ -312,6 +330,10 @@ class CuckooFilter:
                                          _stream_ptr(stream)))
         return i1, i2, fp
 
+    def copyFrom(self, src: "CuckooFilter", stream=None):
+        """Device copy of src's table and counters into this filter."""
+        _check(lib.cf_copy_filter(self._h, src._h, _stream_ptr(stream)))
+
     def saveTable(self, path: str):
         """Header + table image file (cf_save_table)."""
         _check(lib.cf_save_table(self._h, os.fsencode(path)))
@@ -339,6 +361,47 @@ class CuckooFilter:
         _check(lib.cf_compute_packed_tags(self._h, C.c_void_p(k.data_ptr()), k.numel(),
                                           C.c_void_p(out.data_ptr()), _stream_ptr(stream)))
         return out
+
+
+class Batch:
+    """Batched many-filter executor (cf_batch_execute): a window of insert /
+    lookup / delete batches over many filters of one configuration, run with
+    one graph launch and the results of running them one by one."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib.cf_batch_create(C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.cf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def make_ops(ops):
+        """ops: sequence of (filter, op, keys tensor, output tensor or None)."""
+        arr = (cf_batch_op * len(ops))()
+        for i, (f, op, keys, out) in enumerate(ops):
+            arr[i].filter = f._h
+            arr[i].op = op
+            arr[i].d_keys = keys.data_ptr() if keys.numel() else None
+            arr[i].n = keys.numel()
+            arr[i].d_output = out.data_ptr() if out is not None else None
+        return arr
+
+    def execute(self, ops, stream=None, want_occupied=True):
+        arr = ops if isinstance(ops, C.Array) else self.make_ops(ops)
+        occ = np.zeros(len(arr), np.uint64) if want_occupied else None
+        _check(lib.cf_batch_execute(self._h, arr, len(arr), _stream_ptr(stream),
+                                    occ.ctypes.data if occ is not None else None))
+        return occ
 
 
 def write_kmer_file(path: str, keys: np.ndarray):

@@ -133,6 +133,10 @@ int cf_eviction_histogram(cf_filter* f, uint64_t* h_bins, size_t nbins);
 int cf_export_table(cf_filter* f, void* h_dst, size_t bytes);
 int cf_import_table(cf_filter* f, const void* h_src, size_t bytes, size_t occupied);
 
+/* Copies src's table image and counters into dst on the device (same
+ * configuration and bucket count; no reference counterpart). */
+int cf_copy_filter(cf_filter* dst, cf_filter* src, void* stream);
+
 /* getCandidateBuckets on the device for n device keys (the device half of the
  * reference's __host__ __device__ static, CuckooFilter.cuh:366-369). */
 int cf_compute_candidates(cf_filter* f, const void* d_keys, size_t n, uint64_t* d_i1, uint64_t* d_i2,
@@ -141,6 +145,37 @@ int cf_compute_candidates(cf_filter* f, const void* d_keys, size_t n, uint64_t* 
 /* computePackedTagsKernel (CuckooFilter.cuh:1185-1207): d_packed receives n
  * PackedTagType values (uint32_t when bits_per_tag <= 8, else uint64_t). */
 int cf_compute_packed_tags(cf_filter* f, const void* d_keys, size_t n, void* d_packed, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched many-filter execution (no reference counterpart: replaces a loop   */
+/* of the calls above, each of which launches, synchronises and reads the    */
+/* counter back -- CuckooFilter.cuh:457-548).                                */
+/* ------------------------------------------------------------------------ */
+
+enum cf_op { CF_OP_INSERT = 0, CF_OP_LOOKUP = 1, CF_OP_DELETE = 2 };
+
+typedef struct cf_batch_op {
+    cf_filter* filter;    /* target filter */
+    uint32_t op;          /* enum cf_op */
+    uint32_t reserved;    /* 0 */
+    const void* d_keys;   /* device keys of the filter's key width */
+    size_t n;             /* number of keys */
+    bool* d_output;       /* lookup: required; delete: optional; insert: ignored */
+} cf_batch_op;
+
+typedef struct cf_batch cf_batch;
+
+int cf_batch_create(cf_batch** out);
+int cf_batch_destroy(cf_batch* b);
+
+/* Executes ops[0..count) with the results of running them one by one in
+ * array order through cf_insert_many / cf_contains_many / cf_delete_many:
+ * ops on the same filter keep their order, ops on different filters run
+ * together.  Every filter must live on the current device and share one
+ * configuration apart from max_evictions and eviction (which stay per
+ * filter).  Synchronous on `stream`.  h_occupied (optional, count entries)
+ * receives each op's post-op occupancy -- what the plain call returns. */
+int cf_batch_execute(cf_batch* b, const cf_batch_op* ops, size_t count, void* stream, uint64_t* h_occupied);
 
 /* ------------------------------------------------------------------------ */
 /* File formats (no reference counterpart for the table file; the k-mer key  */

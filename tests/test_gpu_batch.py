@@ -1,0 +1,163 @@
+"""Batched many-filter execution (cf_batch_execute) against plain per-batch calls.
+
+A window must give the results of running its batches one by one through the
+plain API (cf_insert_many / cf_contains_many / cf_delete_many):
+  * lookups on given images are bit-exact with the plain calls and with the
+    oracle, also when the captured graph is replayed with eager work between
+    replays (each replay checked);
+  * per-filter order holds inside a window (lookup -> insert -> lookup ->
+    delete -> lookup on one filter);
+  * mixed windows over many filters return the same post-op occupancies,
+    the same delete results, and find every key that is present.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(cf, keys):
+    return cf.keys_to_device(np.ascontiguousarray(keys))
+
+
+def _make_filters(cf, O, caps, seed, load=0.8):
+    """Filters with oracle-built images, and identical twins."""
+    c = cf.make_config(eviction="bfs")
+    oc = O.make_config(**c.as_dict())
+    plain, batched, keysets = [], [], []
+    for i, cap in enumerate(caps):
+        of = O.Filter(oc, cap)
+        keys = O.gen_keys(seed + i, int(of.capacity() * load))
+        occ, _ = of.insert_many(keys)
+        a = cf.CuckooFilter(cap, config=c)
+        a.importTable(of.table_bytes(), occ)
+        b = cf.CuckooFilter(cap, config=c)
+        b.copyFrom(a)
+        plain.append(a)
+        batched.append(b)
+        keysets.append(keys)
+    return plain, batched, keysets
+
+
+def _run_plain(ops):
+    occ = []
+    for f, op, keys, out in ops:
+        if op == 0:
+            occ.append(f.insertMany(keys))
+        elif op == 1:
+            f.containsMany(keys, out=out)
+            occ.append(f.occupiedSlots())
+        else:
+            occ.append(f.deleteMany(keys, out=out))
+    return np.array(occ, dtype=np.uint64)
+
+
+def test_batch_lookups_bit_exact(cf, oracle_mod):
+    import torch
+
+    O = oracle_mod
+    rng = np.random.default_rng(1)
+    caps = [int(2 ** x) for x in rng.uniform(12, 18, size=12)]
+    plain, batched, keysets = _make_filters(cf, O, caps, seed=100)
+    ops_p, ops_b, expect = [], [], []
+    for j in range(200):
+        i = int(rng.integers(len(caps)))
+        n = int(2 ** rng.uniform(0, 14))
+        pos = keysets[i][rng.integers(0, keysets[i].size, n // 2)]
+        neg = O.gen_keys(10_000 + j, n - n // 2)
+        q = np.concatenate([pos, neg])
+        dq = _dev(cf, q)
+        ops_p.append((plain[i], 1, dq, torch.zeros(n, dtype=torch.bool, device="cuda")))
+        ops_b.append((batched[i], 1, dq, torch.zeros(n, dtype=torch.bool, device="cuda")))
+        of = O.Filter(O.make_config(**plain[i].config.as_dict()), caps[i])
+        of.load_table(plain[i].exportTable(), plain[i].occupiedSlots())
+        expect.append(of.contains_many(q))
+    _run_plain(ops_p)
+    b = cf.Batch()
+    arr = b.make_ops(ops_b)
+    for replay in range(4):
+        for _, _, _, out in ops_b:
+            out.zero_()
+        b.execute(arr)
+        for (_, _, _, op), (_, _, _, ob), e in zip(ops_p, ops_b, expect):
+            got = ob.cpu().numpy()
+            np.testing.assert_array_equal(got, op.cpu().numpy(), err_msg=f"replay {replay}")
+            np.testing.assert_array_equal(got, e)
+        # eager work and a synchronisation between replays
+        plain[0].containsMany(ops_p[0][2])
+        torch.cuda.synchronize()
+    b.close()
+
+
+def test_batch_keeps_per_filter_order(cf, oracle_mod):
+    import torch
+
+    f = cf.CuckooFilter(1 << 16, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    g = cf.CuckooFilter(1 << 16, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    k = _dev(cf, oracle_mod.gen_keys(77, 40000))
+    outs = [torch.zeros(k.numel(), dtype=torch.bool, device="cuda") for _ in range(3)]
+    ops = [(f, 1, k, outs[0]), (f, 0, k, None), (f, 1, k, outs[1]), (f, 2, k, None), (f, 1, k, outs[2]),
+           (g, 0, k, None)]
+    b = cf.Batch()
+    occ = b.execute(ops)
+    assert not outs[0].any().item()            # empty filter
+    assert outs[1].all().item()                # after the insert
+    assert not outs[2].any().item()            # after deleting everything again
+    assert list(occ) == [0, 40000, 40000, 0, 0, 40000]
+    assert f.occupiedSlots() == 0 and g.occupiedSlots() == 40000
+    b.close()
+
+
+def test_batch_mixed_window_matches_plain(cf, oracle_mod):
+    import torch
+
+    O = oracle_mod
+    rng = np.random.default_rng(7)
+    caps = [int(2 ** x) for x in rng.uniform(13, 20, size=24)]
+    plain, batched, keysets = _make_filters(cf, O, caps, seed=500, load=0.5)
+    present = [set(map(int, ks)) for ks in keysets]
+    count = [plain[i].occupiedSlots() for i in range(len(caps))]
+    ops_p, ops_b, checks = [], [], []
+    fresh = 0
+    for j in range(400):
+        i = int(rng.integers(len(caps)))
+        kind = rng.choice([0, 1, 2], p=[0.4, 0.5, 0.1])
+        n = int(2 ** rng.uniform(0, 12))
+        if kind == 0:
+            room = int(0.9 * plain[i].capacity()) - count[i]
+            n = max(0, min(n, room))
+            keys = O.gen_keys(90_000 + fresh, n)
+            fresh += 1
+            present[i].update(map(int, keys))
+            count[i] += n
+        elif kind == 1:
+            have = np.fromiter(present[i], dtype=np.uint64) if present[i] else np.zeros(0, np.uint64)
+            pos = have[rng.integers(0, have.size, n // 2)] if have.size else np.zeros(0, np.uint64)
+            keys = np.concatenate([pos, O.gen_keys(70_000 + j, n - pos.size)])
+        else:
+            have = np.fromiter(present[i], dtype=np.uint64)
+            keys = rng.choice(have, size=min(n, have.size), replace=False) if have.size else np.zeros(0, np.uint64)
+            for x in keys:
+                present[i].discard(int(x))
+            count[i] -= keys.size
+        dk = _dev(cf, keys) if keys.size else torch.zeros(0, dtype=torch.int64, device="cuda")
+        out_p = torch.zeros(max(keys.size, 1), dtype=torch.bool, device="cuda") if kind else None
+        out_b = torch.zeros(max(keys.size, 1), dtype=torch.bool, device="cuda") if kind else None
+        ops_p.append((plain[i], int(kind), dk, out_p))
+        ops_b.append((batched[i], int(kind), dk, out_b))
+        want = np.array([int(x) in present[i] for x in keys], dtype=bool) if kind == 1 else None
+        checks.append((int(kind), keys, want))
+    occ_p = _run_plain(ops_p)
+    occ_b = cf.Batch().execute(ops_b)
+    np.testing.assert_array_equal(occ_b, occ_p)
+    for (f, kind, dk, op), (_, _, _, ob), (_, keys, want_pos) in zip(ops_p, ops_b, checks):
+        if kind == 2 and keys.size:
+            assert ob[: keys.size].all().item() and op[: keys.size].all().item()
+        if kind == 1 and keys.size:
+            got = ob[: keys.size].cpu().numpy()
+            assert got[want_pos].all()  # every present key is found
+            # negatives: false-positive counts agree within a binomial band
+            fp_b, fp_p = int(got[~want_pos].sum()), int(op[: keys.size].cpu().numpy()[~want_pos].sum())
+            assert abs(fp_b - fp_p) <= 3 * np.sqrt(max(fp_b + fp_p, 1)) + 2
+    for a, b in zip(plain, batched):
+        assert a.occupiedSlots() == b.occupiedSlots() == b.countOccupiedSlots()
