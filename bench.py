@@ -11,9 +11,12 @@ seeded uint64 keys resident in HBM.  One step is one pass of the hot path:
 
 through the C ABI (each call synchronises and reads the counter back, as the
 reference's API does).  `value` is the whole step's key throughput in Mkeys/s;
-per-operation rates come from HIP events on the launch stream.  The dominant
-kernel's roofline uses algorithmic bytes per key from the sequential oracle at
-the same load trajectory (see DESIGN.md for the figures).
+per-operation rates come from HIP events on the launch stream.  `roofline`
+is containsKernel's (the step's longest single launch), `roofline_insert` the
+insert pipeline's; both count algorithmic bytes per key from the sequential
+oracle at the same load trajectory (BYTES_PER_KEY below), and `traffic` is the
+HBM bytes per launch read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+summary in profiles/hbm_traffic_2p28.md.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 """
@@ -22,6 +25,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -34,6 +38,7 @@ LOAD = 0.95
 N_LOOKUP = 1 << 27
 SEED = 42
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_2p28.md")
 
 # Algorithmic bytes per key (uint64 keys, 32-byte buckets, 8-byte words), with
 # bucket reads / word writes per key from the sequential oracle filling a
@@ -46,6 +51,35 @@ BYTES_PER_KEY = {
     "lookup": 8 + 1 + 32 * 1.5279,
     "delete": 8 + 1 + 32 * 1.0550 + 8 * 1.0,
 }
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE + WRITE_SIZE
+    summary (counters scaled by a plain copy of known size), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            text = fh.read()
+    except OSError:
+        return None
+    m = re.search(rf"^traffic {re.escape(kernel)} (\d+)\s*$", text, re.M)
+    return int(m.group(1)) if m else None
+
+
+def _roofline(kernel, bytes_per_key, n, ms):
+    achieved = bytes_per_key * n / (ms / 1e3) / 1e9
+    traffic = pmc_traffic(kernel.split()[0])
+    return {
+        "bound": "hbm",
+        "kernel": kernel,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "algorithmic_bytes": int(bytes_per_key * n),
+        "bytes_per_key": round(bytes_per_key, 2),
+        "ms_per_launch": round(ms, 3),
+    }
 
 
 def log(msg):
@@ -193,6 +227,149 @@ def batched_section(cf, torch, workloads, dev, n_filters=256, n_batches=8192, se
     }
 
 
+def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(0.5, 0.7, 1.0)):
+    """Serving workload: canonical 31-mers of a 2^28-base ACGT sequence (10%
+    of it in duplicated regions), 2-bit packed on the device, in one 2^30-slot
+    filter, plus 64 tenant filters of 2^20..2^24 slots (log-uniform, filled to
+    U(0.3, 0.8)); all 16-bit tags, bucket 16, BFS.  n_requests requests:
+    sizes log-uniform 1..2^18 keys, target the k-mer filter with probability
+    1/2 (else a uniform tenant), ops 80% lookup / 15% insert / 5% delete.
+    Lookups read a per-filter pool that is half Zipf(1.1)-popular present keys
+    and half absent keys; inserts add fresh keys (tenant inserts clipped at
+    95% load); deletes remove the oldest present keys.  Keys are resident in
+    HBM.  The trace first runs as one burst (every request queued at t = 0)
+    to find the saturated throughput, then open-loop with Poisson arrivals at
+    each fraction of it in `loads`; every run starts from the same filter
+    states.  Latency = completion - scheduled arrival."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    cfg = cf.make_config(key_bytes=8, bits_per_tag=16, bucket_size=16, eviction="bfs")
+    K = 31
+    seq_len = 1 << 28
+    seq = workloads.dna_sequence(seq_len, dup_fraction=0.1, regions=1024, seed=seed, device=dev)
+    kmers = cf.encode_kmers(seq, K, canonical=True)
+    del seq
+    n_kmers = kmers.numel()
+    initial = n_kmers * 3 // 4  # inserted up front; the rest feeds insert requests
+    big = cf.CuckooFilter(1 << 30, config=cfg)
+    big.insertMany(kmers[:initial])
+    absent_mask = (1 << 62) - 1  # 2-bit packed 31-mers use the low 62 bits
+
+    def lookup_pool(present, n_present, size, s):
+        hot = present[workloads.zipf_ranks(size // 2, n_present, s=1.1, seed=s, device=dev)]
+        cold = workloads.gen_keys_torch(s, size - size // 2, device=dev) & absent_mask
+        mix = torch.cat([hot, cold])
+        return mix[torch.randperm(size, device=dev)]
+
+    n_tenants = 64
+    caps = (2 ** rng.uniform(20, 24, n_tenants)).astype(np.int64)
+    filters, pools, lpools, hi, lo, cap_n = [big], [kmers], [], [initial], [0], [n_kmers]
+    lpools.append(lookup_pool(kmers, initial, 1 << 24, seed))
+    for t in range(n_tenants):
+        f = cf.CuckooFilter(int(caps[t]), config=cfg)
+        cap = f.capacity()
+        pool = workloads.gen_keys_torch(7000 + t, cap, device=dev)
+        fill = int(cap * rng.uniform(0.3, 0.8))
+        f.insertMany(pool[:fill])
+        filters.append(f)
+        pools.append(pool)
+        lpools.append(lookup_pool(pool, fill, 1 << 20, 7000 + t))
+        hi.append(fill)
+        lo.append(0)
+        cap_n.append(int(0.95 * cap))
+    snapshots = []
+    for f in filters:
+        c = cf.CuckooFilter(f.capacity(), config=cfg)
+        c.copyFrom(f)
+        snapshots.append(c)
+    ring = torch.empty(1 << 27, dtype=torch.bool, device=dev)
+    ring_at = 0
+    reqs = np.zeros(n_requests, cf.REQUEST_DTYPE)
+    kinds = rng.choice(3, size=n_requests, p=[0.15, 0.8, 0.05])
+    sizes = (2 ** rng.uniform(0, 18, n_requests)).astype(np.int64)
+    target = np.where(rng.random(n_requests) < 0.5, 0, 1 + rng.integers(0, n_tenants, n_requests))
+    clipped = 0
+    total_keys = 0
+    for j in range(n_requests):
+        i, kind, n = int(target[j]), int(kinds[j]), int(sizes[j])
+        out = 0
+        if kind == 1:  # lookup (cf.OP_LOOKUP)
+            pool = lpools[i]
+            start = int(rng.integers(0, pool.numel() - n + 1))
+            keys = pool.data_ptr() + 8 * start
+            if ring_at + n > ring.numel():
+                ring_at = 0
+            out = ring.data_ptr() + ring_at
+            ring_at += n
+        elif kind == 0:  # insert
+            m = max(0, min(n, cap_n[i] - hi[i])) if i else min(n, cap_n[0] - hi[0])
+            clipped += n - m
+            n = m
+            keys = pools[i].data_ptr() + 8 * hi[i]
+            hi[i] += n
+        else:  # delete the oldest present keys
+            n = min(n, hi[i] - lo[i])
+            keys = pools[i].data_ptr() + 8 * lo[i]
+            lo[i] += n
+        reqs[j] = (filters[i]._h.value, kind, 0, keys, n, out)
+        total_keys += n
+    op_keys = {name: int(reqs["n"][reqs["op"] == k].sum()) for k, name in enumerate(("insert", "lookup", "delete"))}
+
+    def restore():
+        for f, c in zip(filters, snapshots):
+            f.copyFrom(c)
+        torch.cuda.synchronize()
+
+    def run(srv, arrivals):
+        restore()
+        srv.reset_stats()
+        res = srv.replay(reqs, arrivals)
+        assert (res["status"] == 0).all(), "serving request failed"
+        st = srv.stats()
+        t_first = int(res["arrival_ns"].min())
+        span = (int(res["done_ns"].max()) - t_first) / 1e9
+        lat = (res["done_ns"] - res["arrival_ns"]).astype(np.float64) / 1e6
+        return {
+            "mkeys_s": round(total_keys / span / 1e6, 1),
+            "requests_s": round(n_requests / span, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "p999_ms": round(float(np.percentile(lat, 99.9)), 3),
+            "max_ms": round(float(lat.max()), 3),
+            "windows": st["windows"],
+            "mean_window_keys": round(st["keys"] / max(st["windows"], 1)),
+            "mean_window_levels": round(st["levels"] / max(st["windows"], 1), 2),
+            "max_queue": st["max_queue"],
+        }
+
+    out = {
+        "workload": ("canonical 31-mers of a 2^28-base sequence (10% duplicated regions) in a 2^30-slot filter "
+                     "+ 64 tenants of 2^20..2^24 slots; requests log-uniform 1..2^18 keys, 80/15/5 "
+                     "lookup/insert/delete, lookups Zipf(1.1) present + 50% absent; keys resident in HBM"),
+        "requests": n_requests,
+        "keys": total_keys,
+        "keys_by_op": op_keys,
+        "insert_keys_clipped": clipped,
+        "kmers": n_kmers,
+    }
+    with cf.Server() as srv:
+        run(srv, np.zeros(n_requests))  # warm-up: plans, graphs, arenas
+        sat = run(srv, np.zeros(n_requests))
+        out["saturated"] = sat
+        peak = sat["mkeys_s"] * 1e6
+        mean_keys = total_keys / n_requests
+        for frac in loads:
+            rate = frac * peak / mean_keys  # requests per second
+            arr = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+            arr -= arr[0]
+            out[f"load_{int(frac * 100)}pct"] = dict(run(srv, arr), offered_mkeys_s=round(frac * peak / 1e6, 1))
+    for x in filters + snapshots:
+        x.close()
+    del kmers, pools, lpools, ring
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +379,7 @@ def main():
     ap.add_argument("--sorted", action="store_true", help="also time insertManySorted")
     ap.add_argument("--no-roofline", action="store_true", help="skip the measured gather/CAS roofline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-filter comparison")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the cuckoo-filter hot path has no multi-GPU leg in this build "
@@ -271,21 +449,16 @@ def main():
         "insert_ms": avg["insert"], "lookup_ms": avg["lookup"], "delete_ms": avg["delete"],
     }
     units = {"insert": n_ins, "lookup": N_LOOKUP, "delete": n_del}
-    dominant = max(avg, key=avg.get)
-    achieved = BYTES_PER_KEY[dominant] * units[dominant] / (avg[dominant] / 1e3) / 1e9
-    roofline = {
-        "bound": "hbm",
-        "kernel": {"insert": "insertKernel", "lookup": "containsKernel", "delete": "deleteKernel"}[dominant],
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "bytes_per_key": round(BYTES_PER_KEY[dominant], 2),
-    }
     for op in ("insert", "lookup", "delete"):
         gbs = BYTES_PER_KEY[op] * units[op] / (avg[op] / 1e3) / 1e9
         per_op[f"{op}_algorithmic_gbs"] = round(gbs, 1)
+    # containsKernel is the step's longest single launch (one launch per
+    # containsMany); insertMany is a pipeline of several launches and is
+    # reported beside it with the same byte accounting.
+    roofline = _roofline("containsKernel", BYTES_PER_KEY["lookup"], N_LOOKUP, avg["lookup"])
+    roofline_insert = _roofline("insertMany pipeline", BYTES_PER_KEY["insert"], n_ins, avg["insert"])
+    roofline_insert["kernels"] = ("packed-entry LSD partition (count + 2 scatter passes), binInsertKernel, "
+                                  "secondaryKernel, bfsStageKernel, dfsStageKernel")
 
     log(f"plain steps done: {ms_step:.2f} ms/step")
     if args.sorted:
@@ -325,6 +498,7 @@ def main():
         },
         "breakdown": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in per_op.items()},
         "roofline": roofline,
+        "roofline_insert": roofline_insert,
         "checks": {"occupied_after_insert": checks.get("occupied"), "expected": n_ins,
                    "occupied_after_delete": checks.get("remaining")},
     }
@@ -334,6 +508,10 @@ def main():
     if not args.no_batched:
         log("batched many-filter window vs plain calls")
         result["batched"] = batched_section(cf, torch, workloads, dev)
+        torch.cuda.empty_cache()
+    if not args.no_serving:
+        log("serving loop: saturated and open-loop latency")
+        result["serving"] = serving_section(cf, torch, workloads, dev)
         torch.cuda.empty_cache()
     if not args.no_roofline:
         import roofline as rl
@@ -345,7 +523,8 @@ def main():
         if g:
             for op in ("insert", "lookup", "delete"):
                 result["breakdown"][f"{op}_frac_of_gather32"] = round(per_op[f"{op}_algorithmic_gbs"] / g, 3)
-            roofline["frac_of_measured_gather32"] = round(achieved / g, 3)
+            for r in (roofline, roofline_insert):
+                r["frac_of_measured_gather32"] = round(r["achieved"] / g, 3)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline")
         result["cpu_baseline"] = cpu_baseline(n_ins, N_LOOKUP, n_del)

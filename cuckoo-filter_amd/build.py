@@ -30,7 +30,7 @@ COMMON = [
     "-DCUCKOO_FILTER_NO_THRUST", f"-I{CSRC}",
 ]
 
-UNITS = [("capi", "capi.hip", []), ("kmer", "kmer.hip", [])] + [
+UNITS = [("capi", "capi.hip", []), ("server", "server.hip", []), ("kmer", "kmer.hip", [])] + [
     (f"inst_p{p}_k{k}", "capi_inst.hip", [f"-DCF_INST_POLICY={p}", f"-DCF_INST_KEY={k}"])
     for p in (0, 1, 2) for k in (4, 8)
 ]

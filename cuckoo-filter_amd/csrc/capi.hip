@@ -16,28 +16,14 @@
 #include <string>
 #include <vector>
 
-#include "capi_table.hpp"
+#include "capi_internal.hpp"
 
 extern "C" int cfk_encode_kmers_device(const void* d_seq, size_t len, uint32_t k, int canonical, uint64_t* d_out,
                                        void* stream);
 
+using namespace cfabi::internal;
+
 namespace {
-
-thread_local std::string g_lastError;
-
-int fail(int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    std::vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_lastError = buf;
-    return code;
-}
-
-int hipFail(hipError_t e, const char* what) {
-    return fail(CF_ERR_HIP, "%s: %s (%d)", what, hipGetErrorString(e), static_cast<int>(e));
-}
 
 uint32_t tagBytes(uint32_t bits) { return bits <= 8 ? 1u : (bits <= 16 ? 2u : 4u); }
 
@@ -98,49 +84,7 @@ SharedWorkspace& sharedWorkspace() {
     return *ws;
 }
 
-/// Makes the filter's device current for the duration of a call.
-class DeviceGuard {
-   public:
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev_) != hipSuccess) prev_ = -1;
-        if (prev_ != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        int cur = -1;
-        if (prev_ >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev_) (void)hipSetDevice(prev_);
-    }
-    DeviceGuard(const DeviceGuard&) = delete;
-    DeviceGuard& operator=(const DeviceGuard&) = delete;
-
-   private:
-    int prev_ = -1;
-};
-
 }  // namespace
-
-struct cf_filter {
-    cf_config cfg{};
-    const cfabi::Ops* ops = nullptr;
-    int device = 0;
-    size_t numBuckets = 0;
-    void* buckets = nullptr;
-    unsigned long long* counters = nullptr;  // [0] occupied, [1] evictions
-    unsigned int* hist = nullptr;            // max_evictions + 2 bins, or nullptr
-
-    size_t tableBytes() const { return numBuckets * ops->bucketBytes; }
-
-    cfabi::AnyView view() const {
-        cfabi::AnyView v;
-        v.words = buckets;
-        v.numBuckets = numBuckets;
-        v.occupied = counters;
-        v.evictions = counters + 1;
-        v.chainHist = hist;
-        v.maxEvictions = cfg.max_evictions;
-        v.evictionPolicy = cfg.eviction;
-        return v;
-    }
-};
 
 namespace {
 
@@ -535,19 +479,7 @@ int cf_batch_execute(cf_batch* b, const cf_batch_op* ops, size_t count, void* st
         if (o.n != 0 && o.d_keys == nullptr) return fail(CF_ERR_INVALID, "op %zu: key pointer is NULL", i);
         if (o.op == CF_OP_LOOKUP && o.n != 0 && o.d_output == nullptr)
             return fail(CF_ERR_INVALID, "op %zu: lookup needs an output", i);
-        cuckoo::BatchOpDesc& d = b->descs[i];
-        d.words = o.filter->buckets;
-        d.numBuckets = o.filter->numBuckets;
-        d.occupied = o.filter->counters;
-        d.evictions = o.filter->counters + 1;
-        d.chainHist = o.filter->hist;
-        d.keys = o.d_keys;
-        d.out = o.op == CF_OP_INSERT ? nullptr : o.d_output;
-        d.n = o.n;
-        d.kind = o.op;
-        d.maxEvictions = o.filter->cfg.max_evictions;
-        d.evictionPolicy = o.filter->cfg.eviction;
-        d.pad = 0;
+        b->descs[i] = describe(o.filter, o.op, o.d_keys, o.n, o.d_output);
     }
     void** state = nullptr;
     for (auto& [k, st] : b->executors) {

@@ -92,6 +92,16 @@ struct Entry {
         if (*state == nullptr) *state = new cuckoo::BatchExecutor<Cfg>();
         return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->execute(ops, n, s, occupiedOut);
     }
+    static hipError_t batchLaunch(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t s) {
+        if (*state == nullptr) *state = new cuckoo::BatchExecutor<Cfg>();
+        return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->launch(ops, n, s);
+    }
+    static const unsigned long long* batchResults(const void* state) {
+        return static_cast<const cuckoo::BatchExecutor<Cfg>*>(state)->results();
+    }
+    static uint32_t batchLevels(const void* state) {
+        return static_cast<const cuckoo::BatchExecutor<Cfg>*>(state)->levelsOfLastLaunch();
+    }
     static void batchRelease(void* state) { delete static_cast<cuckoo::BatchExecutor<Cfg>*>(state); }
 
     static cfabi::Ops make() {
@@ -111,6 +121,9 @@ struct Entry {
         o.candidates = &candidates;
         o.packedTags = &packedTags;
         o.batchExecute = &batchExecute;
+        o.batchLaunch = &batchLaunch;
+        o.batchResults = &batchResults;
+        o.batchLevels = &batchLevels;
         o.batchRelease = &batchRelease;
         return o;
     }

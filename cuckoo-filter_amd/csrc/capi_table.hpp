@@ -41,6 +41,11 @@ struct Ops {
     // batched many-filter execution; *state is this configuration's executor (created on first use)
     hipError_t (*batchExecute)(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t,
                                uint64_t* occupiedOut);
+    // asynchronous form: launches on the stream; batchResults gives the
+    // post-op occupancies once the stream has passed the launch
+    hipError_t (*batchLaunch)(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t);
+    const unsigned long long* (*batchResults)(const void* state);
+    uint32_t (*batchLevels)(const void* state);
     void (*batchRelease)(void* state);
 };
 

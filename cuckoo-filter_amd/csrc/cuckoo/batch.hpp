@@ -4,17 +4,20 @@
 // launch + synchronisation + counter read-back per batch.
 //
 // Ordering: results must equal running the batches one by one through the
-// plain API.  Batches on different filters commute; batches on the same
-// filter do not.  A batch's *level* is the number of earlier batches in the
-// window that target the same filter, and the levels run in order, each as
-// one grid-stride kernel over fixed-size chunks of all of its batches' keys
-// (the batches of one level all target different filters).  After each level
-// a small kernel records every batch's filter occupancy, which is exactly the
-// value the plain insertMany / deleteMany call would have returned.
+// plain API.  Batches on different filters commute; on one filter, lookups
+// commute with each other but not with inserts or deletes.  Every batch gets
+// a *level*: a lookup runs one level after the last earlier insert/delete on
+// its filter, an insert/delete one level after every earlier batch on its
+// filter.  The levels run in order, each as one grid-stride kernel over
+// fixed-size chunks of all of its batches' keys.  After each level a small
+// kernel records every batch's filter occupancy, which is exactly the value
+// the plain insertMany / deleteMany call would have returned.
 //
-// All descriptors live in device buffers that the kernels index by level, so
-// the captured graph stays valid from one window to the next as long as the
-// level count and buffer capacities do not change.
+// A window is one device buffer: a header, the batch descriptors, the
+// chunks and the level tables, uploaded with one copy from pinned memory.
+// The kernels find everything through the header, so a captured graph
+// depends only on the level count and stays valid for every later window
+// with that many levels; graphs are cached per level count.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,6 +25,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <unordered_map>
 #include <vector>
 
@@ -57,14 +61,37 @@ struct BatchChunk {
 
 inline constexpr uint32_t kBatchChunk = 8192;
 inline constexpr int kBatchBlock = 256;
+inline constexpr uint32_t kMaxGraphLevels = 64;  // deeper windows launch level by level
+
+/// Start of a window buffer; offsets are bytes from the start of the buffer.
+struct alignas(64) WindowHeader {
+    uint32_t numOps;
+    uint32_t numChunks;
+    uint32_t numLevels;
+    uint32_t pad;
+    uint64_t opsOff;         // BatchOpDesc[numOps]
+    uint64_t chunksOff;      // BatchChunk[numChunks], grouped by level
+    uint64_t levelOpsOff;    // uint32_t[numOps]: op indices grouped by level
+    uint64_t chunkStartOff;  // uint32_t[numLevels + 1]
+    uint64_t opStartOff;     // uint32_t[numLevels + 1]
+};
+
+namespace detail {
+template <typename T>
+__device__ __forceinline__ const T* windowPart(const unsigned char* w, uint64_t off) {
+    return reinterpret_cast<const T*>(w + off);
+}
+}  // namespace detail
 
 template <typename Config>
-__global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(const BatchOpDesc* __restrict__ ops,
-                                                                const BatchChunk* __restrict__ chunks,
-                                                                const uint32_t* __restrict__ levelChunkStart,
+__global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(const unsigned char* __restrict__ window,
                                                                 uint32_t level) {
     using Ops = FilterOps<Config>;
     using Key = typename Config::KeyType;
+    const auto* hdr = reinterpret_cast<const WindowHeader*>(window);
+    const auto* ops = detail::windowPart<BatchOpDesc>(window, hdr->opsOff);
+    const auto* chunks = detail::windowPart<BatchChunk>(window, hdr->chunksOff);
+    const auto* levelChunkStart = detail::windowPart<uint32_t>(window, hdr->chunkStartOff);
     const uint32_t c0 = levelChunkStart[level];
     const uint32_t c1 = levelChunkStart[level + 1];
     for (uint32_t c = c0 + blockIdx.x; c < c1; c += gridDim.x) {
@@ -112,10 +139,12 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(const BatchOpDes
 
 /// results[op] = occupancy of the op's filter, for the ops of one level.
 template <typename T = uint32_t>
-__global__ __launch_bounds__(256) void batchSnapshotKernel(const BatchOpDesc* __restrict__ ops,
-                                                           const T* __restrict__ levelOps,
-                                                           const uint32_t* __restrict__ levelOpStart,
-                                                           uint32_t level, unsigned long long* __restrict__ results) {
+__global__ __launch_bounds__(256) void batchSnapshotKernel(const unsigned char* __restrict__ window, uint32_t level,
+                                                           unsigned long long* __restrict__ results) {
+    const auto* hdr = reinterpret_cast<const WindowHeader*>(window);
+    const auto* ops = detail::windowPart<BatchOpDesc>(window, hdr->opsOff);
+    const auto* levelOps = detail::windowPart<T>(window, hdr->levelOpsOff);
+    const auto* levelOpStart = detail::windowPart<uint32_t>(window, hdr->opStartOff);
     const uint32_t a = levelOpStart[level];
     const uint32_t b = levelOpStart[level + 1];
     for (uint32_t i = a + blockIdx.x * blockDim.x + threadIdx.x; i < b; i += gridDim.x * blockDim.x) {
@@ -124,8 +153,13 @@ __global__ __launch_bounds__(256) void batchSnapshotKernel(const BatchOpDesc* __
     }
 }
 
-/// Host side: plans a window into levels and chunks, keeps the device
-/// buffers and the captured graph between windows.
+/// Host side: plans windows into levels and chunks and launches them.
+///
+/// launch() is asynchronous; its occupancies (results()) are valid once the
+/// stream has passed the launch.  The executor's staging buffers are reused
+/// by the next launch(), so a caller must not call launch() again before the
+/// previous launch has completed on the device (execute() synchronises;
+/// callers that keep windows in flight use one executor per window slot).
 template <typename Config>
 class BatchExecutor {
    public:
@@ -138,94 +172,148 @@ class BatchExecutor {
     /// occupiedOut (optional, host) receives each op's post-op occupancy.
     hipError_t execute(const BatchOpDesc* ops, size_t count, hipStream_t stream, uint64_t* occupiedOut) {
         if (count == 0) return hipSuccess;
-        // --- plan: levels per filter, chunks per level ---------------------
-        std::unordered_map<const void*, uint32_t> seen;
-        std::vector<uint32_t> level(count);
-        uint32_t numLevels = 0;
-        for (size_t i = 0; i < count; ++i) {
-            uint32_t& k = seen[ops[i].occupied];
-            level[i] = k++;
-            numLevels = std::max(numLevels, k);
-        }
-        std::vector<std::vector<uint32_t>> byLevel(numLevels);
-        for (size_t i = 0; i < count; ++i) byLevel[level[i]].push_back(static_cast<uint32_t>(i));
-        hChunks_.clear();
-        hLevelOps_.clear();
-        std::vector<uint32_t> chunkStart(numLevels + 1), opStart(numLevels + 1);
-        for (uint32_t l = 0; l < numLevels; ++l) {
-            chunkStart[l] = static_cast<uint32_t>(hChunks_.size());
-            opStart[l] = static_cast<uint32_t>(hLevelOps_.size());
-            for (uint32_t op : byLevel[l]) {
-                hLevelOps_.push_back(op);
-                for (uint64_t off = 0; off < ops[op].n; off += kBatchChunk) {
-                    const uint64_t m = std::min<uint64_t>(kBatchChunk, ops[op].n - off);
-                    hChunks_.push_back(BatchChunk{op, static_cast<uint32_t>(m), off});
-                }
-            }
-        }
-        chunkStart[numLevels] = static_cast<uint32_t>(hChunks_.size());
-        opStart[numLevels] = static_cast<uint32_t>(hLevelOps_.size());
-
-        // --- device buffers (grown only) ------------------------------------
-        hipError_t e;
-        bool moved = false;
-        if ((e = grow(&dOps_, &capOps_, count * sizeof(BatchOpDesc), &moved)) != hipSuccess) return e;
-        if ((e = grow(&dChunks_, &capChunks_, std::max<size_t>(1, hChunks_.size()) * sizeof(BatchChunk), &moved)) !=
-            hipSuccess)
-            return e;
-        if ((e = grow(&dLevelOps_, &capLevelOps_, hLevelOps_.size() * sizeof(uint32_t), &moved)) != hipSuccess) return e;
-        if ((e = grow(&dStarts_, &capStarts_, 2 * (numLevels + 1) * sizeof(uint32_t), &moved)) != hipSuccess) return e;
-        if ((e = grow(&dResults_, &capResults_, count * sizeof(unsigned long long), &moved)) != hipSuccess) return e;
-        uint32_t* dChunkStart = static_cast<uint32_t*>(dStarts_);
-        uint32_t* dOpStart = dChunkStart + (numLevels + 1);
-
-        // one host->device upload of everything the graph reads
-        if ((e = hipMemcpyAsync(dOps_, ops, count * sizeof(BatchOpDesc), hipMemcpyHostToDevice, stream)) != hipSuccess ||
-            (e = hipMemcpyAsync(dChunks_, hChunks_.data(), hChunks_.size() * sizeof(BatchChunk),
-                                hipMemcpyHostToDevice, stream)) != hipSuccess ||
-            (e = hipMemcpyAsync(dLevelOps_, hLevelOps_.data(), hLevelOps_.size() * sizeof(uint32_t),
-                                hipMemcpyHostToDevice, stream)) != hipSuccess ||
-            (e = hipMemcpyAsync(dChunkStart, chunkStart.data(), chunkStart.size() * sizeof(uint32_t),
-                                hipMemcpyHostToDevice, stream)) != hipSuccess ||
-            (e = hipMemcpyAsync(dOpStart, opStart.data(), opStart.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                                stream)) != hipSuccess)
-            return e;
-
-        // --- the graph: numLevels x (level kernel, occupancy snapshot) -----
-        if (moved || graphExec_ == nullptr || graphLevels_ != numLevels) {
-            if ((e = capture(numLevels, dChunkStart, dOpStart)) != hipSuccess) return e;
-        }
-        if ((e = hipGraphLaunch(graphExec_, stream)) != hipSuccess) return e;
+        hipError_t e = launch(ops, count, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return e;
         if (occupiedOut != nullptr) {
-            results_.resize(count);
-            if ((e = hipMemcpyAsync(results_.data(), dResults_, count * sizeof(unsigned long long),
-                                    hipMemcpyDeviceToHost, stream)) != hipSuccess)
-                return e;
-        }
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
-        if (occupiedOut != nullptr) {
-            for (size_t i = 0; i < count; ++i) occupiedOut[i] = results_[i];
+            for (size_t i = 0; i < count; ++i) occupiedOut[i] = results()[i];
         }
         return hipSuccess;
     }
 
-    uint32_t levelsOfLastGraph() const { return graphLevels_; }
+    /// Plans `ops`, uploads the window and launches it on `stream` without
+    /// waiting; results()[i] holds op i's post-op occupancy once the stream
+    /// has passed this launch.
+    hipError_t launch(const BatchOpDesc* ops, size_t count, hipStream_t stream) {
+        if (count == 0) return hipSuccess;
+        const uint32_t numLevels = plan(ops, count);
+        if (stageError_ != hipSuccess) return stageError_;
+        hipError_t e;
+        bool moved = false;
+        if ((e = growDevice(&dWin_, &capWin_, staged_, &moved)) != hipSuccess) return e;
+        if ((e = growDevice(&dResults_, &capResults_, count * sizeof(unsigned long long), &moved)) != hipSuccess)
+            return e;
+        if ((e = growPinned(&hResults_, &capHResults_, count * sizeof(unsigned long long))) != hipSuccess) return e;
+        if (moved) dropGraphs();
+        if ((e = hipMemcpyAsync(dWin_, hStage_, staged_, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
+        const auto* win = static_cast<const unsigned char*>(dWin_);
+        auto* results = static_cast<unsigned long long*>(dResults_);
+        if (numLevels <= kMaxGraphLevels) {
+            if (graphs_[numLevels] == nullptr && (e = capture(numLevels)) != hipSuccess) return e;
+            if ((e = hipGraphLaunch(graphs_[numLevels], stream)) != hipSuccess) return e;
+        } else {
+            for (uint32_t l = 0; l < numLevels; ++l) {
+                batchLevelKernel<Config><<<levelGrid(), kBatchBlock, 0, stream>>>(win, l);
+                batchSnapshotKernel<uint32_t><<<4, 256, 0, stream>>>(win, l, results);
+            }
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        lastLevels_ = numLevels;
+        return hipMemcpyAsync(hResults_, dResults_, count * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream);
+    }
+
+    /// Post-op occupancies of the last launch (pinned host memory).
+    const unsigned long long* results() const { return static_cast<const unsigned long long*>(hResults_); }
+    uint32_t levelsOfLastLaunch() const { return lastLevels_; }
 
     void release() {
-        if (graphExec_ != nullptr) (void)hipGraphExecDestroy(graphExec_);
-        graphExec_ = nullptr;
-        for (void** p : {&dOps_, &dChunks_, &dLevelOps_, &dStarts_, &dResults_}) {
+        dropGraphs();
+        for (void** p : {&dWin_, &dResults_}) {
             if (*p != nullptr) (void)hipFree(*p);
             *p = nullptr;
         }
-        capOps_ = capChunks_ = capLevelOps_ = capStarts_ = capResults_ = 0;
+        for (void** p : {&hStage_, &hResults_}) {
+            if (*p != nullptr) (void)hipHostFree(*p);
+            *p = nullptr;
+        }
+        capWin_ = capResults_ = capStage_ = capHResults_ = 0;
         if (captureStream_ != nullptr) (void)hipStreamDestroy(captureStream_);
         captureStream_ = nullptr;
     }
 
    private:
-    static hipError_t grow(void** p, size_t* cap, size_t want, bool* moved) {
-        want = std::max<size_t>(want, 256);
+    static constexpr size_t align64(size_t x) { return (x + 63) & ~static_cast<size_t>(63); }
+
+    /// Writes the window (header, descriptors, chunks, level tables) into the
+    /// pinned staging buffer; returns the level count.
+    uint32_t plan(const BatchOpDesc* ops, size_t count) {
+        seen_.clear();
+        level_.resize(count);
+        uint32_t numLevels = 0;
+        for (size_t i = 0; i < count; ++i) {
+            auto [it, fresh] = seen_.try_emplace(ops[i].occupied, static_cast<uint32_t>(state_.size()));
+            if (fresh) state_.push_back(FilterLevelsPod{-1, -1});
+            FilterLevelsPod& st = state_[it->second];
+            int32_t l;
+            if (ops[i].kind == static_cast<uint32_t>(BatchOpKind::Lookup)) {
+                l = st.lastMutation + 1;
+                st.maxLevel = std::max(st.maxLevel, l);
+            } else {
+                l = st.maxLevel + 1;
+                st.lastMutation = l;
+                st.maxLevel = l;
+            }
+            level_[i] = static_cast<uint32_t>(l);
+            numLevels = std::max(numLevels, static_cast<uint32_t>(l) + 1);
+        }
+        state_.clear();
+        // counting sort of ops by level, stable in window order
+        opStart_.assign(numLevels + 1, 0);
+        for (size_t i = 0; i < count; ++i) ++opStart_[level_[i] + 1];
+        for (uint32_t l = 0; l < numLevels; ++l) opStart_[l + 1] += opStart_[l];
+        levelOps_.resize(count);
+        {
+            std::vector<uint32_t>& cursor = cursor_;
+            cursor.assign(opStart_.begin(), opStart_.end() - 1);
+            for (size_t i = 0; i < count; ++i) levelOps_[cursor[level_[i]]++] = static_cast<uint32_t>(i);
+        }
+        chunks_.clear();
+        chunkStart_.assign(numLevels + 1, 0);
+        for (uint32_t l = 0; l < numLevels; ++l) {
+            chunkStart_[l] = static_cast<uint32_t>(chunks_.size());
+            for (uint32_t j = opStart_[l]; j < opStart_[l + 1]; ++j) {
+                const uint32_t op = levelOps_[j];
+                for (uint64_t off = 0; off < ops[op].n; off += kBatchChunk) {
+                    const uint64_t m = std::min<uint64_t>(kBatchChunk, ops[op].n - off);
+                    chunks_.push_back(BatchChunk{op, static_cast<uint32_t>(m), off});
+                }
+            }
+        }
+        chunkStart_[numLevels] = static_cast<uint32_t>(chunks_.size());
+
+        WindowHeader h{};
+        h.numOps = static_cast<uint32_t>(count);
+        h.numChunks = static_cast<uint32_t>(chunks_.size());
+        h.numLevels = numLevels;
+        size_t off = align64(sizeof(WindowHeader));
+        h.opsOff = off;
+        off = align64(off + count * sizeof(BatchOpDesc));
+        h.chunksOff = off;
+        off = align64(off + chunks_.size() * sizeof(BatchChunk));
+        h.levelOpsOff = off;
+        off = align64(off + count * sizeof(uint32_t));
+        h.chunkStartOff = off;
+        off = align64(off + (numLevels + 1) * sizeof(uint32_t));
+        h.opStartOff = off;
+        off = align64(off + (numLevels + 1) * sizeof(uint32_t));
+        staged_ = off;
+        stageError_ = growPinned(&hStage_, &capStage_, staged_);
+        if (stageError_ != hipSuccess) {
+            staged_ = 0;
+            return 0;
+        }
+        auto* b = static_cast<unsigned char*>(hStage_);
+        std::memcpy(b, &h, sizeof(h));
+        std::memcpy(b + h.opsOff, ops, count * sizeof(BatchOpDesc));
+        std::memcpy(b + h.chunksOff, chunks_.data(), chunks_.size() * sizeof(BatchChunk));
+        std::memcpy(b + h.levelOpsOff, levelOps_.data(), count * sizeof(uint32_t));
+        std::memcpy(b + h.chunkStartOff, chunkStart_.data(), (numLevels + 1) * sizeof(uint32_t));
+        std::memcpy(b + h.opStartOff, opStart_.data(), (numLevels + 1) * sizeof(uint32_t));
+        return numLevels;
+    }
+
+    static hipError_t growDevice(void** p, size_t* cap, size_t want, bool* moved) {
+        want = std::max<size_t>(want, 4096);
         if (want <= *cap) return hipSuccess;
         want = want + want / 2;
         if (*p != nullptr) {
@@ -242,50 +330,76 @@ class BatchExecutor {
         return e;
     }
 
-    hipError_t capture(uint32_t numLevels, const uint32_t* dChunkStart, const uint32_t* dOpStart) {
-        hipError_t e;
-        if (graphExec_ != nullptr) {
-            (void)hipGraphExecDestroy(graphExec_);
-            graphExec_ = nullptr;
+    static hipError_t growPinned(void** p, size_t* cap, size_t want) {
+        want = std::max<size_t>(want, 4096);
+        if (want <= *cap) return hipSuccess;
+        want = want + want / 2;
+        if (*p != nullptr) {
+            const hipError_t e = hipHostFree(*p);
+            *p = nullptr;
+            *cap = 0;
+            if (e != hipSuccess) return e;
         }
+        const hipError_t e = hipHostMalloc(p, want, hipHostMallocDefault);
+        if (e == hipSuccess) *cap = want;
+        return e;
+    }
+
+    static uint32_t levelGrid() {
+        return detail::gridFor(batchLevelKernel<Config>, static_cast<size_t>(1) << 30, kBatchBlock);
+    }
+
+    void dropGraphs() {
+        for (auto& g : graphs_) {
+            if (g != nullptr) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    }
+
+    hipError_t capture(uint32_t numLevels) {
+        hipError_t e;
         if (captureStream_ == nullptr && (e = hipStreamCreateWithFlags(&captureStream_, hipStreamNonBlocking)) != hipSuccess)
             return e;
-        auto k = batchLevelKernel<Config>;
-        const uint32_t grid = detail::gridFor(k, static_cast<size_t>(1) << 30, kBatchBlock);
-        const auto* ops = static_cast<const BatchOpDesc*>(dOps_);
-        const auto* chunks = static_cast<const BatchChunk*>(dChunks_);
-        const auto* levelOps = static_cast<const uint32_t*>(dLevelOps_);
+        const auto* win = static_cast<const unsigned char*>(dWin_);
         auto* results = static_cast<unsigned long long*>(dResults_);
+        const uint32_t grid = levelGrid();
         if ((e = hipStreamBeginCapture(captureStream_, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
         for (uint32_t l = 0; l < numLevels; ++l) {
-            k<<<grid, kBatchBlock, 0, captureStream_>>>(ops, chunks, dChunkStart, l);
-            batchSnapshotKernel<uint32_t><<<4, 256, 0, captureStream_>>>(ops, levelOps, dOpStart, l, results);
+            batchLevelKernel<Config><<<grid, kBatchBlock, 0, captureStream_>>>(win, l);
+            batchSnapshotKernel<uint32_t><<<4, 256, 0, captureStream_>>>(win, l, results);
         }
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(captureStream_, &graph);
         if (e != hipSuccess) return e;
-        e = hipGraphInstantiate(&graphExec_, graph, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&graphs_[numLevels], graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-            graphExec_ = nullptr;
-            return e;
-        }
-        graphLevels_ = numLevels;
-        return hipSuccess;
+        if (e != hipSuccess) graphs_[numLevels] = nullptr;
+        return e;
     }
 
-    void* dOps_ = nullptr;
-    void* dChunks_ = nullptr;
-    void* dLevelOps_ = nullptr;
-    void* dStarts_ = nullptr;
+    struct FilterLevelsPod {
+        int32_t lastMutation;
+        int32_t maxLevel;
+    };
+
+    // device: the window buffer and the occupancy results
+    void* dWin_ = nullptr;
     void* dResults_ = nullptr;
-    size_t capOps_ = 0, capChunks_ = 0, capLevelOps_ = 0, capStarts_ = 0, capResults_ = 0;
-    std::vector<BatchChunk> hChunks_;
-    std::vector<uint32_t> hLevelOps_;
-    std::vector<unsigned long long> results_;
+    size_t capWin_ = 0, capResults_ = 0;
+    // pinned host: the staged window and the results read back
+    void* hStage_ = nullptr;
+    void* hResults_ = nullptr;
+    size_t capStage_ = 0, capHResults_ = 0, staged_ = 0;
+    hipError_t stageError_ = hipSuccess;
+    // planning scratch (host)
+    std::unordered_map<const void*, uint32_t> seen_;
+    std::vector<FilterLevelsPod> state_;
+    std::vector<uint32_t> level_, opStart_, levelOps_, chunkStart_, cursor_;
+    std::vector<BatchChunk> chunks_;
+    // graphs by level count
     hipStream_t captureStream_ = nullptr;
-    hipGraphExec_t graphExec_ = nullptr;
-    uint32_t graphLevels_ = 0;
+    hipGraphExec_t graphs_[kMaxGraphLevels + 1] = {};
+    uint32_t lastLevels_ = 0;
 };
 
 }  // namespace cuckoo

@@ -1,0 +1,504 @@
+// Serving loop over resident filters: continuous batching of queued requests
+// into windows run by the batched many-filter executor (cuckoo/batch.hpp).
+//
+// Threads: callers submit into a FIFO queue; the launcher thread forms a
+// window from the queue head whenever a window slot is free and launches it
+// on the server's stream; the completer thread waits for windows in launch
+// order, copies host-memory results out and publishes per-request results.
+// One stream runs every window, so windows execute in launch order and the
+// batch executor's per-window ordering gives arrival order per filter.
+// Two slots: one window runs on the device while the next one is formed,
+// staged and queued behind it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <unordered_set>
+#include <utility>
+#include <vector>
+
+#include "capi_internal.hpp"
+
+using namespace cfabi::internal;
+
+namespace {
+
+constexpr int kSlots = 2;
+constexpr size_t kStageAlign = 256;
+
+size_t alignUp(size_t x) { return (x + kStageAlign - 1) & ~(kStageAlign - 1); }
+
+uint64_t nowNs() {
+    return static_cast<uint64_t>(
+        std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+            .count());
+}
+
+struct Pending {
+    cf_request req{};
+    uint64_t ticket = 0;
+    uint64_t arrivalNs = 0;
+    size_t stageKeys = 0;    // staging bytes for keys (host-memory requests)
+    size_t stageOut = 0;     // staging bytes for results (host-memory requests)
+    cf_request_result* sink = nullptr;  // replay: result goes straight here
+    uint64_t* sinkCount = nullptr;      // replay: completions counted here
+};
+
+struct Slot {
+    std::vector<std::pair<const cfabi::Ops*, void*>> executors;  // one per configuration used
+    void* dArena = nullptr;  // device staging
+    void* hArena = nullptr;  // pinned host staging (same layout)
+    hipEvent_t done = nullptr;
+    // the window in this slot
+    std::vector<Pending> reqs;
+    std::vector<cuckoo::BatchOpDesc> descs;
+    std::vector<size_t> keyOff, outOff;  // arena offsets per request (host-memory requests)
+    const cfabi::Ops* ops = nullptr;
+    void* state = nullptr;
+    uint64_t id = 0;
+    uint64_t startNs = 0;
+    int status = CF_OK;
+    uint32_t levels = 0;
+};
+
+}  // namespace
+
+struct cf_server {
+    cf_server_options opt{};
+    size_t arenaBytes = 0;  // per slot
+    hipStream_t stream = nullptr;
+    Slot slots[kSlots];
+
+    std::mutex mu;
+    std::condition_variable cvWork;      // launcher: queue or free slot changed
+    std::condition_variable cvInflight;  // completer: a window was launched
+    std::condition_variable cvDone;      // waiters: results published
+    std::deque<Pending> queue;
+    std::vector<int> freeSlots;
+    std::deque<int> inflight;
+    std::unordered_set<uint64_t> outstanding;
+    std::unordered_map<uint64_t, cf_request_result> finished;
+    uint64_t nextTicket = 1;
+    uint64_t nextWindow = 0;
+    uint64_t submitted = 0;
+    uint64_t completed = 0;
+    bool stopping = false;
+    bool launcherDone = false;
+    cf_server_stats stats{};
+
+    std::thread launcher;
+    std::thread completer;
+
+    void launcherLoop();
+    void completerLoop();
+    int launchWindow(Slot& sl);
+};
+
+namespace {
+
+void* executorFor(Slot& sl, const cfabi::Ops* ops) {
+    for (auto& [k, st] : sl.executors) {
+        if (k == ops) return &st;
+    }
+    sl.executors.emplace_back(ops, nullptr);
+    return &sl.executors.back().second;
+}
+
+}  // namespace
+
+int cf_server::launchWindow(Slot& sl) {
+    // host-memory requests: keys into the pinned arena, one upload
+    size_t keyBytes = 0;
+    sl.keyOff.assign(sl.reqs.size(), 0);
+    sl.outOff.assign(sl.reqs.size(), 0);
+    for (size_t i = 0; i < sl.reqs.size(); ++i) {
+        if (sl.reqs[i].stageKeys == 0) continue;
+        sl.keyOff[i] = keyBytes;
+        keyBytes += sl.reqs[i].stageKeys;
+    }
+    size_t outBytes = keyBytes;
+    for (size_t i = 0; i < sl.reqs.size(); ++i) {
+        if (sl.reqs[i].stageOut == 0) continue;
+        sl.outOff[i] = outBytes;
+        outBytes += sl.reqs[i].stageOut;
+    }
+    auto* h = static_cast<unsigned char*>(sl.hArena);
+    auto* d = static_cast<unsigned char*>(sl.dArena);
+    sl.descs.resize(sl.reqs.size());
+    for (size_t i = 0; i < sl.reqs.size(); ++i) {
+        const cf_request& r = sl.reqs[i].req;
+        const void* keys = r.keys;
+        bool* out = r.output;
+        if (r.flags & CF_REQUEST_HOST_MEMORY) {
+            const size_t kb = static_cast<size_t>(r.n) * r.filter->cfg.key_bytes;
+            if (kb != 0) std::memcpy(h + sl.keyOff[i], r.keys, kb);
+            keys = d + sl.keyOff[i];
+            out = sl.reqs[i].stageOut != 0 ? reinterpret_cast<bool*>(d + sl.outOff[i]) : nullptr;
+        }
+        sl.descs[i] = describe(r.filter, r.op, keys, r.n, out);
+    }
+    hipError_t e = hipSuccess;
+    if (keyBytes != 0) e = hipMemcpyAsync(d, h, keyBytes, hipMemcpyHostToDevice, stream);
+    void** st = static_cast<void**>(executorFor(sl, sl.ops));
+    if (e == hipSuccess) e = sl.ops->batchLaunch(st, sl.descs.data(), sl.descs.size(), stream);
+    sl.state = *st;
+    if (e == hipSuccess && outBytes > keyBytes)
+        e = hipMemcpyAsync(h + keyBytes, d + keyBytes, outBytes - keyBytes, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipEventRecord(sl.done, stream);
+    sl.levels = e == hipSuccess ? sl.ops->batchLevels(sl.state) : 0;
+    if (e != hipSuccess) {
+        const int rc = hipFail(e, "launching a window");
+        (void)hipStreamSynchronize(stream);  // nothing of this window may still use the slot
+        return rc;
+    }
+    return CF_OK;
+}
+
+void cf_server::launcherLoop() {
+    (void)hipSetDevice(opt.device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        cvWork.wait(lk, [&] { return (stopping && queue.empty()) || (!queue.empty() && !freeSlots.empty()); });
+        if (queue.empty()) break;  // stopping and drained
+        const int si = freeSlots.back();
+        freeSlots.pop_back();
+        Slot& sl = slots[si];
+        sl.reqs.clear();
+        sl.ops = queue.front().req.filter->ops;
+        size_t keys = 0;
+        size_t stage = 0;
+        while (!queue.empty() && sl.reqs.size() < opt.max_window_requests) {
+            const Pending& p = queue.front();
+            if (p.req.filter->ops != sl.ops) break;  // one configuration per window
+            if (!sl.reqs.empty() && keys + p.req.n > opt.max_window_keys) break;
+            if (stage + p.stageKeys + p.stageOut > arenaBytes) break;  // waits for the arena
+            keys += p.req.n;
+            stage += p.stageKeys + p.stageOut;
+            sl.reqs.push_back(p);
+            queue.pop_front();
+        }
+        sl.id = nextWindow++;
+        lk.unlock();
+        sl.startNs = nowNs();
+        sl.status = launchWindow(sl);
+        lk.lock();
+        stats.windows += 1;
+        stats.requests += sl.reqs.size();
+        stats.keys += keys;
+        stats.levels += sl.levels;
+        inflight.push_back(si);
+        cvInflight.notify_one();
+    }
+    launcherDone = true;
+    cvInflight.notify_one();
+}
+
+void cf_server::completerLoop() {
+    (void)hipSetDevice(opt.device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        cvInflight.wait(lk, [&] { return !inflight.empty() || launcherDone; });
+        if (inflight.empty()) break;
+        const int si = inflight.front();
+        lk.unlock();
+        Slot& sl = slots[si];
+        int status = sl.status;
+        if (status == CF_OK) {
+            hipError_t e;
+            uint32_t spins = 0;
+            while ((e = hipEventQuery(sl.done)) == hipErrorNotReady) {
+                if (++spins > 2000) std::this_thread::sleep_for(std::chrono::microseconds(20));
+                else std::this_thread::yield();
+            }
+            if (e != hipSuccess) status = hipFail(e, "window completion");
+        }
+        const unsigned long long* occ = status == CF_OK ? sl.ops->batchResults(sl.state) : nullptr;
+        const auto* h = static_cast<const unsigned char*>(sl.hArena);
+        std::vector<cf_request_result> res(sl.reqs.size());
+        for (size_t i = 0; i < sl.reqs.size(); ++i) {
+            const Pending& p = sl.reqs[i];
+            if (status == CF_OK && p.stageOut != 0) std::memcpy(p.req.output, h + sl.outOff[i], p.req.n);
+            cf_request_result& r = res[i];
+            r.status = status;
+            r.window_levels = sl.levels;
+            r.window = sl.id;
+            r.occupied = occ != nullptr ? occ[i] : 0;
+            r.arrival_ns = p.arrivalNs;
+            r.start_ns = sl.startNs;
+        }
+        const uint64_t t = nowNs();
+        lk.lock();
+        for (size_t i = 0; i < sl.reqs.size(); ++i) {
+            res[i].done_ns = t;
+            const Pending& p = sl.reqs[i];
+            if (p.sink != nullptr) {
+                *p.sink = res[i];
+                *p.sinkCount += 1;
+            } else {
+                outstanding.erase(p.ticket);
+                finished.emplace(p.ticket, res[i]);
+            }
+        }
+        completed += sl.reqs.size();
+        sl.reqs.clear();
+        inflight.pop_front();
+        freeSlots.push_back(si);
+        cvWork.notify_one();
+        cvDone.notify_all();
+    }
+}
+
+namespace {
+
+int checkRequest(const cf_server* s, const cf_request* r, size_t arenaBytes, size_t* stageKeys, size_t* stageOut) {
+    if (r == nullptr) return fail(CF_ERR_INVALID, "request is NULL");
+    if (r->filter == nullptr || r->filter->ops == nullptr) return fail(CF_ERR_INVALID, "request filter is NULL");
+    if (r->filter->device != s->opt.device)
+        return fail(CF_ERR_INVALID, "request filter lives on device %d, the server on %d", r->filter->device,
+                    s->opt.device);
+    if (r->op > CF_OP_DELETE) return fail(CF_ERR_INVALID, "unknown operation %u", r->op);
+    if (r->n != 0 && r->keys == nullptr) return fail(CF_ERR_INVALID, "key pointer is NULL");
+    if (r->op == CF_OP_LOOKUP && r->n != 0 && r->output == nullptr)
+        return fail(CF_ERR_INVALID, "lookup needs an output");
+    *stageKeys = 0;
+    *stageOut = 0;
+    if (r->flags & ~CF_REQUEST_HOST_MEMORY) return fail(CF_ERR_INVALID, "unknown request flags 0x%x", r->flags);
+    if (r->flags & CF_REQUEST_HOST_MEMORY) {
+        *stageKeys = alignUp(static_cast<size_t>(r->n) * r->filter->cfg.key_bytes);
+        const bool wantsOut = r->op == CF_OP_LOOKUP || (r->op == CF_OP_DELETE && r->output != nullptr);
+        *stageOut = wantsOut ? alignUp(static_cast<size_t>(r->n)) : 0;
+        if (*stageKeys + *stageOut > arenaBytes)
+            return fail(CF_ERR_INVALID,
+                        "host-memory request of %llu keys needs %zu staging bytes; a window has %zu (raise "
+                        "staging_bytes or pass device memory)",
+                        static_cast<unsigned long long>(r->n), *stageKeys + *stageOut, arenaBytes);
+    }
+    return CF_OK;
+}
+
+int enqueue(cf_server* s, const cf_request* r, uint64_t arrivalNs, cf_request_result* sink, uint64_t* sinkCount,
+            uint64_t* ticket) {
+    size_t sk = 0, so = 0;
+    const int rc = checkRequest(s, r, s->arenaBytes, &sk, &so);
+    if (rc != CF_OK) return rc;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) return fail(CF_ERR_INVALID, "server is stopping");
+    Pending p;
+    p.req = *r;
+    p.ticket = s->nextTicket++;
+    p.arrivalNs = arrivalNs;
+    p.stageKeys = sk;
+    p.stageOut = so;
+    p.sink = sink;
+    p.sinkCount = sinkCount;
+    if (sink == nullptr) s->outstanding.insert(p.ticket);
+    s->queue.push_back(p);
+    s->submitted += 1;
+    s->stats.max_queue = std::max<uint64_t>(s->stats.max_queue, s->queue.size());
+    if (ticket != nullptr) *ticket = p.ticket;
+    s->cvWork.notify_one();
+    return CF_OK;
+}
+
+void freeServer(cf_server* s) {
+    (void)hipSetDevice(s->opt.device);
+    for (Slot& sl : s->slots) {
+        for (auto& [ops, st] : sl.executors) {
+            if (st != nullptr) ops->batchRelease(st);
+        }
+        if (sl.dArena != nullptr) (void)hipFree(sl.dArena);
+        if (sl.hArena != nullptr) (void)hipHostFree(sl.hArena);
+        if (sl.done != nullptr) (void)hipEventDestroy(sl.done);
+    }
+    if (s->stream != nullptr) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t cf_server_now_ns(void) { return nowNs(); }
+
+int cf_server_options_init(cf_server_options* o) {
+    if (o == nullptr) return fail(CF_ERR_INVALID, "options is NULL");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    o->device = dev;
+    o->max_window_requests = 4096;
+    o->max_window_keys = 1ull << 22;
+    o->staging_bytes = 0;
+    return CF_OK;
+}
+
+int cf_server_create(const cf_server_options* o, cf_server** out) {
+    if (out == nullptr) return fail(CF_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    cf_server_options opt{};
+    if (o != nullptr) {
+        opt = *o;
+    } else {
+        cf_server_options_init(&opt);
+    }
+    if (opt.max_window_requests == 0 || opt.max_window_keys == 0)
+        return fail(CF_ERR_INVALID, "window limits must be positive");
+    auto* s = new (std::nothrow) cf_server();
+    if (s == nullptr) return fail(CF_ERR_ALLOC, "host allocation failed");
+    s->opt = opt;
+    DeviceGuard g(opt.device);
+    hipError_t e = hipSuccess;
+    size_t staging = opt.staging_bytes;
+    if (staging == 0) {
+        size_t freeB = 0, totalB = 0;
+        e = hipMemGetInfo(&freeB, &totalB);
+        if (e != hipSuccess) {
+            delete s;
+            return hipFail(e, "reading free device memory");
+        }
+        staging = std::min<size_t>(512ull << 20, freeB / 16);
+    }
+    s->arenaBytes = std::max<size_t>(kStageAlign, (staging / kSlots) & ~(kStageAlign - 1));
+    s->stats.staging_bytes = s->arenaBytes * kSlots;
+    e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    for (int i = 0; i < kSlots && e == hipSuccess; ++i) {
+        Slot& sl = s->slots[i];
+        e = hipMalloc(&sl.dArena, s->arenaBytes);
+        if (e == hipSuccess) e = hipHostMalloc(&sl.hArena, s->arenaBytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        s->freeSlots.push_back(kSlots - 1 - i);
+    }
+    if (e != hipSuccess) {
+        const int rc = hipFail(e, "server set-up");
+        freeServer(s);
+        return rc;
+    }
+    try {
+        s->launcher = std::thread([s] { s->launcherLoop(); });
+        s->completer = std::thread([s] { s->completerLoop(); });
+    } catch (...) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stopping = true;
+            if (!s->launcher.joinable()) s->launcherDone = true;
+        }
+        s->cvWork.notify_all();
+        if (s->launcher.joinable()) s->launcher.join();
+        s->cvInflight.notify_all();
+        if (s->completer.joinable()) s->completer.join();
+        freeServer(s);
+        return fail(CF_ERR_ALLOC, "starting the server threads failed");
+    }
+    *out = s;
+    return CF_OK;
+}
+
+int cf_server_destroy(cf_server* s) {
+    if (s == nullptr) return CF_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cvWork.notify_all();
+    s->launcher.join();   // drains the queue into windows
+    s->completer.join();  // retires every launched window
+    freeServer(s);
+    return CF_OK;
+}
+
+int cf_server_submit(cf_server* s, const cf_request* r, uint64_t* ticket) {
+    if (s == nullptr) return fail(CF_ERR_INVALID, "server is NULL");
+    return enqueue(s, r, nowNs(), nullptr, nullptr, ticket);
+}
+
+int cf_server_wait(cf_server* s, uint64_t ticket, cf_request_result* out) {
+    if (s == nullptr) return fail(CF_ERR_INVALID, "server is NULL");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->finished.count(ticket) == 0 && s->outstanding.count(ticket) == 0)
+        return fail(CF_ERR_INVALID, "unknown ticket %llu", static_cast<unsigned long long>(ticket));
+    s->cvDone.wait(lk, [&] { return s->finished.count(ticket) != 0; });
+    auto it = s->finished.find(ticket);
+    if (out != nullptr) *out = it->second;
+    s->finished.erase(it);
+    return CF_OK;
+}
+
+int cf_server_poll(cf_server* s, uint64_t ticket, cf_request_result* out) {
+    if (s == nullptr) return fail(CF_ERR_INVALID, "server is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->finished.find(ticket);
+    if (it == s->finished.end()) {
+        if (s->outstanding.count(ticket) == 0)
+            return fail(CF_ERR_INVALID, "unknown ticket %llu", static_cast<unsigned long long>(ticket));
+        return 0;
+    }
+    if (out != nullptr) *out = it->second;
+    s->finished.erase(it);
+    return 1;
+}
+
+int cf_server_drain(cf_server* s) {
+    if (s == nullptr) return fail(CF_ERR_INVALID, "server is NULL");
+    std::unique_lock<std::mutex> lk(s->mu);
+    const uint64_t target = s->submitted;
+    s->cvDone.wait(lk, [&] { return s->completed >= target; });
+    return CF_OK;
+}
+
+int cf_server_stats_get(cf_server* s, cf_server_stats* out) {
+    if (s == nullptr || out == nullptr) return fail(CF_ERR_INVALID, "server or out is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    return CF_OK;
+}
+
+int cf_server_stats_reset(cf_server* s) {
+    if (s == nullptr) return fail(CF_ERR_INVALID, "server is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    const uint64_t staging = s->stats.staging_bytes;
+    s->stats = cf_server_stats{};
+    s->stats.staging_bytes = staging;
+    return CF_OK;
+}
+
+int cf_server_replay(cf_server* s, const cf_request* reqs, const double* arrival_s, size_t count,
+                     cf_request_result* results) {
+    if (s == nullptr) return fail(CF_ERR_INVALID, "server is NULL");
+    if (count == 0) return CF_OK;
+    if (reqs == nullptr || arrival_s == nullptr || results == nullptr)
+        return fail(CF_ERR_INVALID, "reqs, arrival_s and results must not be NULL");
+    for (size_t i = 0; i < count; ++i) {
+        if (!(arrival_s[i] >= 0) || (i > 0 && arrival_s[i] < arrival_s[i - 1]))
+            return fail(CF_ERR_INVALID, "arrival_s must be non-negative and non-decreasing (index %zu)", i);
+        size_t sk, so;
+        const int rc = checkRequest(s, &reqs[i], s->arenaBytes, &sk, &so);
+        if (rc != CF_OK) return fail(rc, "request %zu: %s", i, g_lastError.c_str());
+    }
+    uint64_t done = 0;  // guarded by s->mu
+    const uint64_t t0 = nowNs();
+    for (size_t i = 0; i < count; ++i) {
+        const uint64_t at = t0 + static_cast<uint64_t>(arrival_s[i] * 1e9);
+        for (uint64_t t = nowNs(); t < at; t = nowNs()) {
+            if (at - t > 200000) std::this_thread::sleep_for(std::chrono::nanoseconds(at - t - 100000));
+        }
+        const int rc = enqueue(s, &reqs[i], at, &results[i], &done, nullptr);
+        if (rc != CF_OK) {
+            // the ones already queued still write into results / done: wait for them
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cvDone.wait(lk, [&] { return done >= i; });
+            return rc;
+        }
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cvDone.wait(lk, [&] { return done >= count; });
+    return CF_OK;
+}
+
+}  // extern "C"

@@ -45,6 +45,23 @@ class cf_batch_op(C.Structure):
     ]
 
 
+CF_REQUEST_HOST_MEMORY = 1
+
+# cf_request / cf_request_result / cf_server_options / cf_server_stats as numpy
+# records (same layout as the C structs), so request lists need no per-item
+# ctypes objects.
+REQUEST_DTYPE = np.dtype([("filter", "u8"), ("op", "u4"), ("flags", "u4"), ("keys", "u8"), ("n", "u8"),
+                          ("output", "u8")])
+RESULT_DTYPE = np.dtype([("status", "i4"), ("window_levels", "u4"), ("window", "u8"), ("occupied", "u8"),
+                         ("arrival_ns", "u8"), ("start_ns", "u8"), ("done_ns", "u8")])
+SERVER_OPTIONS_DTYPE = np.dtype([("device", "i4"), ("max_window_requests", "u4"), ("max_window_keys", "u8"),
+                                 ("staging_bytes", "u8")])
+SERVER_STATS_DTYPE = np.dtype([(k, "u8") for k in ("windows", "requests", "keys", "levels", "staging_bytes",
+                                                    "max_queue")])
+assert REQUEST_DTYPE.itemsize == 40 and RESULT_DTYPE.itemsize == 48
+assert SERVER_OPTIONS_DTYPE.itemsize == 24 and SERVER_STATS_DTYPE.itemsize == 48
+
+
 class cf_config(C.Structure):
     _fields_ = [
         ("key_bytes", C.c_uint32),
@@ -106,6 +123,17 @@ def _load():
         "cf_kmer_file_count": (i32, [C.c_char_p, C.POINTER(sz)]),
         "cf_kmer_file_read": (i32, [C.c_char_p, P, sz]),
         "cf_encode_kmers": (i32, [P, sz, C.c_uint32, i32, P, P]),
+        "cf_server_options_init": (i32, [P]),
+        "cf_server_create": (i32, [P, C.POINTER(P)]),
+        "cf_server_destroy": (i32, [P]),
+        "cf_server_submit": (i32, [P, P, C.POINTER(C.c_uint64)]),
+        "cf_server_wait": (i32, [P, C.c_uint64, P]),
+        "cf_server_poll": (i32, [P, C.c_uint64, P]),
+        "cf_server_drain": (i32, [P]),
+        "cf_server_stats_get": (i32, [P, P]),
+        "cf_server_stats_reset": (i32, [P]),
+        "cf_server_now_ns": (C.c_uint64, []),
+        "cf_server_replay": (i32, [P, P, P, sz, P]),
     }
     del u64p, u32p
     for name, (res, args) in sig.items():
@@ -402,6 +430,119 @@ class Batch:
         _check(lib.cf_batch_execute(self._h, arr, len(arr), _stream_ptr(stream),
                                     occ.ctypes.data if occ is not None else None))
         return occ
+
+
+def _ptr(buf) -> int:
+    """Address of a torch tensor or numpy array (0 for None / empty)."""
+    if buf is None:
+        return 0
+    if isinstance(buf, np.ndarray):
+        return buf.ctypes.data if buf.size else 0
+    return buf.data_ptr() if buf.numel() else 0
+
+
+class Server:
+    """Serving loop (cf_server_*): requests against resident filters, queued in
+    arrival order and run in continuously formed batched windows.
+
+    A request is (filter, op, keys, out, host): keys / out are device tensors,
+    or numpy arrays with host=True (staged through the server's device arena).
+    Results equal running the requests one by one in arrival order."""
+
+    def __init__(self, device=None, max_window_requests=None, max_window_keys=None, staging_bytes=None):
+        o = np.zeros(1, SERVER_OPTIONS_DTYPE)
+        _check(lib.cf_server_options_init(o.ctypes.data))
+        if device is not None:
+            o["device"] = device
+        if max_window_requests is not None:
+            o["max_window_requests"] = max_window_requests
+        if max_window_keys is not None:
+            o["max_window_keys"] = max_window_keys
+        if staging_bytes is not None:
+            o["staging_bytes"] = staging_bytes
+        h = C.c_void_p()
+        _check(lib.cf_server_create(o.ctypes.data, C.byref(h)))
+        self._h = h
+        self._alive = {}  # ticket -> buffers kept alive until the request completes
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(lib.cf_server_destroy(self._h))
+            self._h = None
+            self._alive.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def make_requests(reqs):
+        """reqs: sequence of (filter, op, keys, out, host) -> REQUEST_DTYPE array."""
+        arr = np.zeros(len(reqs), REQUEST_DTYPE)
+        for i, (f, op, keys, out, host) in enumerate(reqs):
+            n = keys.size if isinstance(keys, np.ndarray) else keys.numel()
+            arr[i] = (f._h.value, op, CF_REQUEST_HOST_MEMORY if host else 0, _ptr(keys), n, _ptr(out))
+        return arr
+
+    def submit(self, f, op, keys, out=None, host=False) -> int:
+        r = self.make_requests([(f, op, keys, out, host)])
+        t = C.c_uint64()
+        _check(lib.cf_server_submit(self._h, r.ctypes.data, C.byref(t)))
+        self._alive[t.value] = (keys, out)
+        return t.value
+
+    def _result(self, res):
+        return {k: int(res[k][0]) for k in RESULT_DTYPE.names}
+
+    def wait(self, ticket: int) -> dict:
+        res = np.zeros(1, RESULT_DTYPE)
+        _check(lib.cf_server_wait(self._h, ticket, res.ctypes.data))
+        self._alive.pop(ticket, None)
+        return self._result(res)
+
+    def poll(self, ticket: int):
+        res = np.zeros(1, RESULT_DTYPE)
+        rc = lib.cf_server_poll(self._h, ticket, res.ctypes.data)
+        if rc < 0:
+            _check(rc)
+        if rc == 0:
+            return None
+        self._alive.pop(ticket, None)
+        return self._result(res)
+
+    def drain(self):
+        _check(lib.cf_server_drain(self._h))
+
+    def stats(self) -> dict:
+        st = np.zeros(1, SERVER_STATS_DTYPE)
+        _check(lib.cf_server_stats_get(self._h, st.ctypes.data))
+        return {k: int(st[k][0]) for k in SERVER_STATS_DTYPE.names}
+
+    def reset_stats(self):
+        _check(lib.cf_server_stats_reset(self._h))
+
+    def replay(self, reqs: np.ndarray, arrival_s: np.ndarray) -> np.ndarray:
+        """Open-loop replay of a REQUEST_DTYPE array at arrival_s (seconds from
+        the start, non-decreasing); returns a RESULT_DTYPE array."""
+        reqs = np.ascontiguousarray(reqs, dtype=REQUEST_DTYPE)
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        if arr.shape != (reqs.size,):
+            raise ValueError("arrival_s must have one entry per request")
+        res = np.zeros(reqs.size, RESULT_DTYPE)
+        _check(lib.cf_server_replay(self._h, reqs.ctypes.data, arr.ctypes.data, reqs.size, res.ctypes.data))
+        return res
+
+
+def server_now_ns() -> int:
+    return int(lib.cf_server_now_ns())
 
 
 def write_kmer_file(path: str, keys: np.ndarray):

@@ -12,6 +12,9 @@ Runs the kernels of libcuckoo_roofline.so with HIP events:
 Each figure is algorithmic bytes (the bytes named above) per second.
 
     python cuckoo-filter_amd/roofline.py            # prints one JSON object
+    python cuckoo-filter_amd/roofline.py --copy-only
+        # only the 1 GiB copy: the known-size kernel that scales the
+        # FETCH_SIZE / WRITE_SIZE counters to bytes
 """
 from __future__ import annotations
 
@@ -89,17 +92,29 @@ def measure(table_bytes=512 << 20, n=1 << 26, reps=5, block=256, big_factor=4) -
         del tab
         torch.cuda.empty_cache()
 
-    cb = 1 << 30
-    src = torch.empty(cb // 4, dtype=torch.int32, device=dev)
-    dst = torch.empty_like(src)
-
-    def copy():
-        rc = L.rl_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), cb, block, s)
-        assert rc == 0, rc
-    t = _time(torch, copy, reps)
-    res["copy_gbs"] = round(2 * cb / t / 1e9, 1)
+    res["copy_gbs"] = measure_copy(reps=reps, block=block)
     return res
 
 
+def measure_copy(copy_bytes=1 << 30, reps=5, block=256) -> float:
+    """GB/s (bytes read + bytes written) of a streaming copy of copy_bytes."""
+    import torch
+
+    L = _lib()
+    dev = torch.device("cuda:0")
+    s = C.c_void_p(int(torch.cuda.current_stream().cuda_stream))
+    src = torch.empty(copy_bytes // 4, dtype=torch.int32, device=dev)
+    dst = torch.empty_like(src)
+
+    def copy():
+        rc = L.rl_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), copy_bytes, block, s)
+        assert rc == 0, rc
+    t = _time(torch, copy, reps)
+    return round(2 * copy_bytes / t / 1e9, 1)
+
+
 if __name__ == "__main__":
-    print(json.dumps(measure()), flush=True)
+    if "--copy-only" in sys.argv[1:]:
+        print(json.dumps({"copy_bytes": 1 << 30, "copy_gbs": measure_copy()}), flush=True)
+    else:
+        print(json.dumps(measure()), flush=True)

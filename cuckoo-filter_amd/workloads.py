@@ -56,3 +56,39 @@ def _low32(z):
     lo = z & 0xFFFFFFFF
     lo = torch.where(lo >= (1 << 31), lo - (1 << 32), lo)
     return lo.to(torch.int32)
+
+
+def dna_sequence(length: int, dup_fraction: float = 0.1, regions: int = 1024, seed: int = 42, device="cuda"):
+    """ASCII ACGT sequence of `length` uniform random bases in which
+    `dup_fraction` of the bases are copies of earlier stretches: `regions`
+    stretches of equal length, each copied from a random earlier position to a
+    random later one (the repeats a k-mer set of a real genome holds)."""
+    import torch
+
+    g = torch.Generator(device=device).manual_seed(seed)
+    codes = torch.randint(0, 4, (length,), dtype=torch.uint8, device=device, generator=g)
+    seq = torch.tensor([65, 67, 71, 84], dtype=torch.uint8, device=device)[codes.long()]
+    del codes
+    span = int(length * dup_fraction) // max(regions, 1)
+    if span > 0:
+        gen = torch.Generator().manual_seed(seed + 1)
+        for _ in range(regions):
+            src = int(torch.randint(0, length - 2 * span, (1,), generator=gen))
+            dst = int(torch.randint(src + span, length - span + 1, (1,), generator=gen))
+            seq[dst:dst + span] = seq[src:src + span].clone()
+    return seq
+
+
+def zipf_ranks(m: int, n_items: int, s: float = 1.1, seed: int = 42, device="cuda"):
+    """m item indices in [0, n_items) with Zipf(s) popularity: rank r >= 1 is
+    drawn by inverting the continuous power-law tail P(R > r) = r^-(s-1)
+    (r = u^(-1/(s-1)), u uniform in (0, 1]), folded into [0, n_items), and
+    scattered over the items by an odd multiplier mod 2^64, so the hottest
+    items are not neighbours in memory."""
+    import torch
+
+    g = torch.Generator(device=device).manual_seed(seed)
+    u = torch.rand(m, dtype=torch.float64, device=device, generator=g).clamp_(min=1e-300)
+    r = torch.pow(u, -1.0 / (s - 1.0)).clamp_(max=2.0 ** 62)
+    rank = (r.to(torch.int64) - 1) % n_items
+    return torch.remainder(rank * _s64(C_STEP), n_items)

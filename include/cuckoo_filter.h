@@ -178,6 +178,88 @@ int cf_batch_destroy(cf_batch* b);
 int cf_batch_execute(cf_batch* b, const cf_batch_op* ops, size_t count, void* stream, uint64_t* h_occupied);
 
 /* ------------------------------------------------------------------------ */
+/* Serving loop (no counterpart in the reference; its nearest analogue is    */
+/* the synchronous one-worker IPC server, include/CuckooFilterIPC.cuh:200-307) */
+/* ------------------------------------------------------------------------ */
+
+/* A server runs requests against resident filters with continuous batching.
+ * Requests queue in arrival order; a launcher thread moves everything queued
+ * (within the window limits) into the next free window slot and runs it as
+ * one batched window (the cf_batch_execute semantics) on the server's
+ * stream; a completer thread retires windows in order and publishes the
+ * results.  Two window slots keep one window forming while the other runs.
+ * Results equal running the requests one by one, in arrival order, through
+ * the plain calls.  Requests with CF_REQUEST_HOST_MEMORY pass host buffers:
+ * their keys and results travel through a device staging arena, and such a
+ * request is admitted to a window only when its bytes fit in the arena
+ * (admission waits, in arrival order, for running windows to free space).
+ * A filter used by a server must not be used by other calls meanwhile. */
+typedef struct cf_server cf_server;
+
+#define CF_REQUEST_HOST_MEMORY 1u /* keys / output are host memory */
+
+typedef struct cf_server_options {
+    int32_t device;               /* default: the current device */
+    uint32_t max_window_requests; /* default 4096 */
+    uint64_t max_window_keys;     /* default 1 << 22 (a larger single request runs alone) */
+    uint64_t staging_bytes;       /* device staging arena for host-memory requests;
+                                     0 = min(512 MiB, 1/16 of free HBM at creation) */
+} cf_server_options;
+
+typedef struct cf_request {
+    cf_filter* filter;
+    uint32_t op;      /* cf_op */
+    uint32_t flags;   /* CF_REQUEST_* */
+    const void* keys; /* n keys of the filter's key width */
+    uint64_t n;
+    bool* output;     /* lookup: required; delete: optional; insert: unused */
+} cf_request;
+
+typedef struct cf_request_result {
+    int32_t status;         /* CF_OK or a CF_ERR_* code */
+    uint32_t window_levels; /* levels of the window it ran in */
+    uint64_t window;        /* id of that window (0, 1, ...) */
+    uint64_t occupied;      /* filter occupancy after the request (what insertMany /
+                               deleteMany return) */
+    uint64_t arrival_ns;    /* steady clock (cf_server_now_ns): submission, or the
+                               scheduled arrival under cf_server_replay */
+    uint64_t start_ns;      /* its window was launched */
+    uint64_t done_ns;       /* its window had completed and its results were out */
+} cf_request_result;
+
+typedef struct cf_server_stats {
+    uint64_t windows;
+    uint64_t requests;
+    uint64_t keys;
+    uint64_t levels;        /* summed over windows */
+    uint64_t staging_bytes; /* arena size in use */
+    uint64_t max_queue;     /* deepest request queue seen */
+} cf_server_stats;
+
+int cf_server_options_init(cf_server_options* o);
+int cf_server_create(const cf_server_options* o, cf_server** out);
+/* Finishes every submitted request, then stops the threads and frees. */
+int cf_server_destroy(cf_server* s);
+/* Queues one request; *ticket names it for cf_server_wait / cf_server_poll.
+ * The request's buffers must stay valid until it has completed. */
+int cf_server_submit(cf_server* s, const cf_request* r, uint64_t* ticket);
+/* Blocks until the request has completed, fills *out and forgets the ticket. */
+int cf_server_wait(cf_server* s, uint64_t ticket, cf_request_result* out);
+/* 1 (and *out filled, ticket forgotten) if complete, 0 if still pending. */
+int cf_server_poll(cf_server* s, uint64_t ticket, cf_request_result* out);
+/* Blocks until every request submitted so far has completed. */
+int cf_server_drain(cf_server* s);
+int cf_server_stats_get(cf_server* s, cf_server_stats* out);
+int cf_server_stats_reset(cf_server* s);
+uint64_t cf_server_now_ns(void);
+/* Open-loop replay: submits reqs[i] at arrival_s[i] seconds (non-decreasing)
+ * after the call starts, from this thread, then waits for all of them;
+ * results[i].arrival_ns is the scheduled arrival, so queueing behind a late
+ * submission still counts in the latency. */
+int cf_server_replay(cf_server* s, const cf_request* reqs, const double* arrival_s, size_t count,
+                     cf_request_result* results);
+
+/* ------------------------------------------------------------------------ */
 /* File formats (no reference counterpart for the table file; the k-mer key  */
 /* file is the reference's src/kmc_to_binary.cpp:1-7 format).                */
 /* ------------------------------------------------------------------------ */

@@ -7,6 +7,7 @@
   python-xxhash vectors.
 """
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -89,3 +90,46 @@ def test_host_candidates_match_golden(cfm, golden):
         np.testing.assert_array_equal(i1, golden[name + "/i1"], err_msg=name)
         np.testing.assert_array_equal(i2, golden[name + "/i2"], err_msg=name)
         np.testing.assert_array_equal(fp, golden[name + "/fp"], err_msg=name)
+
+
+def test_struct_layouts_match_header(cfm, tmp_path):
+    """The Python records mirror the C structs field by field (gcc on the header)."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    structs = {
+        "cf_request": cfm.REQUEST_DTYPE,
+        "cf_request_result": cfm.RESULT_DTYPE,
+        "cf_server_options": cfm.SERVER_OPTIONS_DTYPE,
+        "cf_server_stats": cfm.SERVER_STATS_DTYPE,
+    }
+    ctypes_structs = {"cf_config": cfm.cf_config, "cf_batch_op": cfm.cf_batch_op}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cuckoo_filter.h"', "int main(void) {"]
+    for name, dt in structs.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for fld in dt.names:
+            lines.append(f'printf("{name} {fld} %zu\\n", offsetof({name}, {fld}));')
+    for name, st in ctypes_structs.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for fld, _ in st._fields_:
+            lines.append(f'printf("{name} {fld} %zu\\n", offsetof({name}, {fld}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        name, fld, val = line.split()
+        got[(name, fld)] = int(val)
+    for name, dt in structs.items():
+        assert got[(name, "size")] == dt.itemsize, name
+        for fld in dt.names:
+            assert got[(name, fld)] == dt.fields[fld][1], (name, fld)
+    for name, st in ctypes_structs.items():
+        assert got[(name, "size")] == ctypes.sizeof(st), name
+        for fld, _ in st._fields_:
+            assert got[(name, fld)] == getattr(st, fld).offset, (name, fld)

@@ -9,6 +9,8 @@ counter falls by exactly that many."""
 import numpy as np
 import pytest
 
+from conftest import assert_table_count
+
 pytestmark = pytest.mark.gpu
 
 
@@ -31,7 +33,8 @@ def test_hbm_scale_fill_lookup_delete(cf, mode, monkeypatch):
     print(f"inserted {occ} of {n}", flush=True)
     assert occ >= n - 10  # BFS+DFS(500) leaves essentially no failures at 95% load
     img = f.exportTable()
-    assert int(np.count_nonzero(img.view(np.uint16))) == occ == f.countOccupiedSlots()
+    table = assert_table_count(f, occ, n)
+    assert int(np.count_nonzero(img.view(np.uint16))) == table
     found = f.containsMany(pos)
     missing = int((~found).sum().item())
     assert missing <= n - occ

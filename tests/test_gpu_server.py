@@ -1,0 +1,229 @@
+"""Serving loop (cf_server_*) against the plain calls and the oracle.
+
+The serving contract: every request gets the result it would get if the
+requests ran one by one, in arrival order, through the plain API.
+  * lookups are pure: on given images they are bit-exact with the oracle
+    whether a request runs alone (submit + wait) or in the mix (a replayed
+    burst that the server batches into shared windows);
+  * inserts / deletes in a mixed stream: each request's post-op occupancy
+    equals the plain call's, deletes of present keys all succeed, every
+    present key is found at its point in the order, and the false-positive
+    count of negatives agrees with the plain run within a binomial band;
+  * host-memory requests go through the staging arena; an arena smaller than
+    the stream makes admission wait, without changing results.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_table_count
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_filters(cf, O, caps, seed, load):
+    c = cf.make_config(eviction="bfs")
+    oc = O.make_config(**c.as_dict())
+    fs, ofs, keysets = [], [], []
+    for i, cap in enumerate(caps):
+        of = O.Filter(oc, cap)
+        keys = O.gen_keys(seed + i, int(of.capacity() * load))
+        occ, _ = of.insert_many(keys)
+        f = cf.CuckooFilter(cap, config=c)
+        f.importTable(of.table_bytes(), occ)
+        fs.append(f)
+        ofs.append(of)
+        keysets.append(keys)
+    return fs, ofs, keysets
+
+
+def _lookup_stream(cf, O, fs, keysets, rng, count, max_log2=13, host_every=3):
+    import torch
+
+    reqs, expect_idx, queries = [], [], []
+    for j in range(count):
+        i = int(rng.integers(len(fs)))
+        n = int(2 ** rng.uniform(0, max_log2))
+        pos = keysets[i][rng.integers(0, keysets[i].size, n // 2)]
+        q = np.concatenate([pos, O.gen_keys(50_000 + j, n - n // 2)])
+        host = j % host_every == 0
+        if host:
+            keys, out = q, np.zeros(n, dtype=bool)
+        else:
+            keys, out = cf.keys_to_device(q), torch.zeros(n, dtype=torch.bool, device="cuda")
+        reqs.append((fs[i], cf.OP_LOOKUP, keys, out, host))
+        expect_idx.append(i)
+        queries.append(q)
+    return reqs, expect_idx, queries
+
+
+def _out_np(out):
+    return out if isinstance(out, np.ndarray) else out.cpu().numpy()
+
+
+def test_server_lookups_alone_and_in_mix_bit_exact(cf, oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(3)
+    caps = [int(2 ** x) for x in rng.uniform(12, 18, size=10)]
+    fs, ofs, keysets = _oracle_filters(cf, O, caps, seed=300, load=0.85)
+    reqs, idx, queries = _lookup_stream(cf, O, fs, keysets, rng, 300)
+    expect = [ofs[i].contains_many(q) for i, q in zip(idx, queries)]
+    with cf.Server() as srv:
+        # alone: one request in flight at a time
+        for (f, op, keys, out, host), e in zip(reqs[:60], expect[:60]):
+            t = srv.submit(f, op, keys, out, host)
+            r = srv.wait(t)
+            assert r["status"] == 0
+            np.testing.assert_array_equal(_out_np(out), e)
+            assert r["occupied"] == f.occupiedSlots()
+        # in the mix: the whole stream as one burst
+        for _, _, _, out, _ in reqs:
+            if isinstance(out, np.ndarray):
+                out[:] = False
+            else:
+                out.zero_()
+        arr = srv.make_requests(reqs)
+        res = srv.replay(arr, np.zeros(len(reqs)))
+        assert (res["status"] == 0).all()
+        st = srv.stats()
+        assert st["requests"] == 60 + len(reqs)
+        assert st["windows"] < 60 + len(reqs)  # the burst shares windows
+        for (f, _, _, out, _), e, r in zip(reqs, expect, res):
+            np.testing.assert_array_equal(_out_np(out), e)
+            assert int(r["occupied"]) == f.occupiedSlots()
+        assert (res["done_ns"] >= res["start_ns"]).all() and (res["start_ns"] >= res["arrival_ns"]).all()
+
+
+def test_server_keeps_arrival_order_per_filter(cf, oracle_mod):
+    import torch
+
+    f = cf.CuckooFilter(1 << 16, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    g = cf.CuckooFilter(1 << 16, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    k = cf.keys_to_device(oracle_mod.gen_keys(78, 30000))
+    outs = [torch.zeros(k.numel(), dtype=torch.bool, device="cuda") for _ in range(4)]
+    reqs = [(f, cf.OP_LOOKUP, k, outs[0], False), (f, cf.OP_INSERT, k, None, False),
+            (g, cf.OP_INSERT, k, None, False), (f, cf.OP_LOOKUP, k, outs[1], False),
+            (f, cf.OP_LOOKUP, k, outs[2], False), (f, cf.OP_DELETE, k, None, False),
+            (f, cf.OP_LOOKUP, k, outs[3], False)]
+    with cf.Server() as srv:
+        res = srv.replay(srv.make_requests(reqs), np.zeros(len(reqs)))
+    assert (res["status"] == 0).all()
+    assert not outs[0].any().item()
+    assert outs[1].all().item() and outs[2].all().item()
+    assert not outs[3].any().item()
+    assert list(res["occupied"]) == [0, 30000, 30000, 30000, 30000, 0, 0]
+    assert f.occupiedSlots() == 0 and g.occupiedSlots() == 30000
+
+
+@pytest.mark.parametrize("staging", [None, 1 << 20])
+def test_server_mixed_stream_matches_plain(cf, oracle_mod, staging):
+    import torch
+
+    O = oracle_mod
+    rng = np.random.default_rng(11 if staging else 12)
+    caps = [int(2 ** x) for x in rng.uniform(13, 19, size=16)]
+    plain, _, keysets = _oracle_filters(cf, O, caps, seed=700, load=0.5)
+    served = []
+    for a in plain:
+        b = cf.CuckooFilter(a.capacity(), config=a.config)
+        b.copyFrom(a)
+        served.append(b)
+    present = [set(map(int, ks)) for ks in keysets]
+    count = [a.occupiedSlots() for a in plain]
+    plan = []
+    for j in range(300):
+        i = int(rng.integers(len(caps)))
+        kind = int(rng.choice([0, 1, 2], p=[0.3, 0.55, 0.15]))
+        n = int(2 ** rng.uniform(0, 11))
+        if kind == 0:
+            n = max(0, min(n, int(0.9 * plain[i].capacity()) - count[i]))
+            keys = O.gen_keys(120_000 + j, n)
+            present[i].update(map(int, keys))
+            count[i] += n
+            want = None
+        elif kind == 1:
+            have = np.fromiter(present[i], dtype=np.uint64)
+            pos = have[rng.integers(0, have.size, n // 2)] if have.size else np.zeros(0, np.uint64)
+            keys = np.concatenate([pos, O.gen_keys(150_000 + j, n - pos.size)])
+            want = np.array([int(x) in present[i] for x in keys], dtype=bool)
+        else:
+            have = np.fromiter(present[i], dtype=np.uint64)
+            keys = rng.choice(have, size=min(n, have.size), replace=False) if have.size else np.zeros(0, np.uint64)
+            for x in keys:
+                present[i].discard(int(x))
+            count[i] -= keys.size
+            want = None
+        plan.append((i, kind, keys, want, j % 2 == 0))
+    # plain, one by one
+    occ_plain, out_plain = [], []
+    for i, kind, keys, _, _ in plan:
+        dk = cf.keys_to_device(keys) if keys.size else torch.zeros(0, dtype=torch.int64, device="cuda")
+        if kind == 0:
+            occ_plain.append(plain[i].insertMany(dk))
+            out_plain.append(None)
+        elif kind == 1:
+            o = plain[i].containsMany(dk) if keys.size else torch.zeros(0, dtype=torch.bool, device="cuda")
+            out_plain.append(o.cpu().numpy())
+            occ_plain.append(plain[i].occupiedSlots())
+        else:
+            o = torch.zeros(max(keys.size, 1), dtype=torch.bool, device="cuda")
+            occ_plain.append(plain[i].deleteMany(dk, out=o[: keys.size]))
+            out_plain.append(o[: keys.size].cpu().numpy())
+    # served: Poisson arrivals over ~30 ms, half the requests in host memory
+    reqs = []
+    for i, kind, keys, _, host in plan:
+        if host:
+            k = keys.copy()
+            out = np.zeros(keys.size, dtype=bool) if kind else None
+        else:
+            k = cf.keys_to_device(keys) if keys.size else torch.zeros(0, dtype=torch.int64, device="cuda")
+            out = torch.zeros(max(keys.size, 1), dtype=torch.bool, device="cuda") if kind else None
+        reqs.append((served[i], kind, k, out, host))
+    arrivals = np.cumsum(rng.exponential(1e-4, size=len(reqs)))
+    with cf.Server(staging_bytes=staging) as srv:
+        res = srv.replay(srv.make_requests(reqs), arrivals)
+        st = srv.stats()
+    assert (res["status"] == 0).all()
+    np.testing.assert_array_equal(res["occupied"], np.array(occ_plain, dtype=np.uint64))
+    for (i, kind, keys, want, _), (_, _, _, out, _), op in zip(plan, reqs, out_plain):
+        if not keys.size or kind == 0:
+            continue
+        got = _out_np(out)[: keys.size]
+        if kind == 2:
+            assert got.all() and op.all()
+        else:
+            assert got[want].all()
+            fp_s, fp_p = int(got[~want].sum()), int(op[~want].sum())
+            assert abs(fp_s - fp_p) <= 3 * np.sqrt(max(fp_s + fp_p, 1)) + 2
+    for a, b in zip(plain, served):
+        assert a.occupiedSlots() == b.occupiedSlots()
+        assert_table_count(b, b.occupiedSlots(), b.capacity())
+    if staging:
+        assert st["staging_bytes"] <= staging
+
+
+def test_server_ticket_api_and_errors(cf, oracle_mod):
+    import torch
+
+    f = cf.CuckooFilter(1 << 14, key_bytes=8, bits_per_tag=16)
+    keys = oracle_mod.gen_keys(5, 1000)
+    with cf.Server(staging_bytes=64 << 10) as srv:
+        t1 = srv.submit(f, cf.OP_INSERT, keys, None, host=True)
+        out = np.zeros(keys.size, dtype=bool)
+        t2 = srv.submit(f, cf.OP_LOOKUP, keys, out, host=True)
+        srv.drain()
+        r2 = srv.poll(t2)
+        assert r2 is not None and r2["status"] == 0 and out.all()
+        assert srv.wait(t1)["occupied"] == 1000
+        with pytest.raises(cf.CuckooError):
+            srv.wait(t1)  # already collected
+        with pytest.raises(cf.CuckooError):
+            srv.poll(10 ** 9)  # never issued
+        big = oracle_mod.gen_keys(6, 10000)  # 80 KB of keys > a 32 KiB window arena
+        with pytest.raises(cf.CuckooError, match="staging"):
+            srv.submit(f, cf.OP_LOOKUP, big, np.zeros(big.size, dtype=bool), host=True)
+        with pytest.raises(cf.CuckooError):
+            srv.submit(f, cf.OP_LOOKUP, keys, None, host=True)  # lookup without output
+        # device memory is not staged: the same request passes on the device
+        dk = cf.keys_to_device(big)
+        dout = torch.zeros(big.size, dtype=torch.bool, device="cuda")
+        assert srv.wait(srv.submit(f, cf.OP_LOOKUP, dk, dout))["status"] == 0
