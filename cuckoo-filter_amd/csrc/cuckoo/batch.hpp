@@ -9,12 +9,13 @@
 // a *level*: a lookup runs one level after the last earlier insert/delete on
 // its filter, an insert/delete one level after every earlier batch on its
 // filter.  The levels run in order, each as one grid-stride kernel over
-// fixed-size chunks of all of its batches' keys.  After each level a small
-// kernel records every batch's filter occupancy, which is exactly the value
+// 512-key chunks of all of its batches' keys.  The last workgroup to finish a
+// level records every batch's filter occupancy, which is exactly the value
 // the plain insertMany / deleteMany call would have returned.
 //
 // A window is one device buffer: a header, the batch descriptors, the
-// chunks and the level tables, uploaded with one copy from pinned memory.
+// level tables and per-level completion counters, uploaded with one copy
+// from pinned memory.
 // The kernels find everything through the header, so a captured graph
 // depends only on the level count and stays valid for every later window
 // with that many levels; graphs are cached per level count.
@@ -52,51 +53,62 @@ struct BatchOpDesc {
     uint32_t pad;
 };
 
-/// A run of at most kBatchChunk keys of one batch.
-struct BatchChunk {
-    uint32_t op;
-    uint32_t count;
-    uint64_t offset;
-};
-
-inline constexpr uint32_t kBatchChunk = 8192;
+inline constexpr uint32_t kBatchChunk = 512;  // keys per workgroup step: two per thread
 inline constexpr int kBatchBlock = 256;
 inline constexpr uint32_t kMaxGraphLevels = 64;  // deeper windows launch level by level
 
 /// Start of a window buffer; offsets are bytes from the start of the buffer.
 struct alignas(64) WindowHeader {
     uint32_t numOps;
-    uint32_t numChunks;
     uint32_t numLevels;
-    uint32_t pad;
-    uint64_t opsOff;         // BatchOpDesc[numOps]
-    uint64_t chunksOff;      // BatchChunk[numChunks], grouped by level
-    uint64_t levelOpsOff;    // uint32_t[numOps]: op indices grouped by level
-    uint64_t chunkStartOff;  // uint32_t[numLevels + 1]
-    uint64_t opStartOff;     // uint32_t[numLevels + 1]
+    uint32_t pad[2];
+    uint64_t opsOff;       // BatchOpDesc[numOps]
+    uint64_t levelOpsOff;  // uint32_t[numOps]: op indices grouped by level, window order inside a level
+    uint64_t chunkCumOff;  // uint32_t[numOps + 1]: kBatchChunk-key chunks before each levelOps position
+    uint64_t opStartOff;   // uint32_t[numLevels + 1]: first levelOps position of each level
+    uint64_t doneOff;      // uint32_t[numLevels]: workgroups finished per level (uploaded as zero)
 };
 
 namespace detail {
 template <typename T>
-__device__ __forceinline__ const T* windowPart(const unsigned char* w, uint64_t off) {
-    return reinterpret_cast<const T*>(w + off);
+__device__ __forceinline__ T* windowPart(unsigned char* w, uint64_t off) {
+    return reinterpret_cast<T*>(w + off);
 }
 }  // namespace detail
 
+/// One level: a grid-stride loop over the level's chunks (each workgroup
+/// finds its batch by binary search over the chunk counts), then the last
+/// workgroup to finish records results[op] = the op's filter occupancy for
+/// every op of the level.
 template <typename Config>
-__global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(const unsigned char* __restrict__ window,
-                                                                uint32_t level) {
+__global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* __restrict__ window, uint32_t level,
+                                                                unsigned long long* __restrict__ results) {
     using Ops = FilterOps<Config>;
     using Key = typename Config::KeyType;
     const auto* hdr = reinterpret_cast<const WindowHeader*>(window);
-    const auto* ops = detail::windowPart<BatchOpDesc>(window, hdr->opsOff);
-    const auto* chunks = detail::windowPart<BatchChunk>(window, hdr->chunksOff);
-    const auto* levelChunkStart = detail::windowPart<uint32_t>(window, hdr->chunkStartOff);
-    const uint32_t c0 = levelChunkStart[level];
-    const uint32_t c1 = levelChunkStart[level + 1];
-    for (uint32_t c = c0 + blockIdx.x; c < c1; c += gridDim.x) {
-        const BatchChunk ch = chunks[c];
-        const BatchOpDesc& o = ops[ch.op];
+    const auto* ops = detail::windowPart<const BatchOpDesc>(window, hdr->opsOff);
+    const auto* levelOps = detail::windowPart<const uint32_t>(window, hdr->levelOpsOff);
+    const auto* chunkCum = detail::windowPart<const uint32_t>(window, hdr->chunkCumOff);
+    const auto* opStart = detail::windowPart<const uint32_t>(window, hdr->opStartOff);
+    auto* done = detail::windowPart<uint32_t>(window, hdr->doneOff);
+    const uint32_t p0 = opStart[level];
+    const uint32_t p1 = opStart[level + 1];
+    const uint32_t g1 = chunkCum[p1];
+    for (uint32_t g = chunkCum[p0] + blockIdx.x; g < g1; g += gridDim.x) {
+        // last position j in [p0, p1) with chunkCum[j] <= g (skips empty batches)
+        uint32_t lo = p0;
+        uint32_t hi = p1 - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (chunkCum[mid] <= g) {
+                lo = mid;
+            } else {
+                hi = mid - 1;
+            }
+        }
+        const BatchOpDesc& o = ops[levelOps[lo]];
+        const uint64_t offset = static_cast<uint64_t>(g - chunkCum[lo]) * kBatchChunk;
+        const uint32_t count = static_cast<uint32_t>(o.n - offset < kBatchChunk ? o.n - offset : kBatchChunk);
         DeviceView<Config> v;
         v.words = static_cast<typename DeviceView<Config>::Word*>(o.words);
         v.numBuckets = o.numBuckets;
@@ -105,13 +117,13 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(const unsigned c
         v.chainHist = o.chainHist;
         v.maxEvictions = o.maxEvictions;
         v.evictionPolicy = o.evictionPolicy;
-        const Key* keys = static_cast<const Key*>(o.keys) + ch.offset;
+        const Key* keys = static_cast<const Key*>(o.keys) + offset;
         uint32_t ok = 0;
         uint32_t ev = 0;
         if (o.kind == static_cast<uint32_t>(BatchOpKind::Lookup)) {
-            for (uint32_t i = threadIdx.x; i < ch.count; i += kBatchBlock) o.out[ch.offset + i] = Ops::contains(v, keys[i]);
+            for (uint32_t i = threadIdx.x; i < count; i += kBatchBlock) o.out[offset + i] = Ops::contains(v, keys[i]);
         } else if (o.kind == static_cast<uint32_t>(BatchOpKind::Insert)) {
-            for (uint32_t i = threadIdx.x; i < ch.count; i += kBatchBlock) {
+            for (uint32_t i = threadIdx.x; i < count; i += kBatchBlock) {
                 uint32_t chain;
                 const bool s = Ops::insert(v, keys[i], chain);
                 ok += s;
@@ -121,10 +133,10 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(const unsigned c
                 }
             }
         } else {
-            for (uint32_t i = threadIdx.x; i < ch.count; i += kBatchBlock) {
+            for (uint32_t i = threadIdx.x; i < count; i += kBatchBlock) {
                 const bool s = Ops::remove(v, keys[i]);
                 ok += s;
-                if (o.out != nullptr) o.out[ch.offset + i] = s;
+                if (o.out != nullptr) o.out[offset + i] = s;
             }
         }
         // uniform across the block: every thread takes the same branch per chunk
@@ -135,25 +147,30 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(const unsigned c
             }
         }
     }
-}
-
-/// results[op] = occupancy of the op's filter, for the ops of one level.
-template <typename T = uint32_t>
-__global__ __launch_bounds__(256) void batchSnapshotKernel(const unsigned char* __restrict__ window, uint32_t level,
-                                                           unsigned long long* __restrict__ results) {
-    const auto* hdr = reinterpret_cast<const WindowHeader*>(window);
-    const auto* ops = detail::windowPart<BatchOpDesc>(window, hdr->opsOff);
-    const auto* levelOps = detail::windowPart<T>(window, hdr->levelOpsOff);
-    const auto* levelOpStart = detail::windowPart<uint32_t>(window, hdr->opStartOff);
-    const uint32_t a = levelOpStart[level];
-    const uint32_t b = levelOpStart[level + 1];
-    for (uint32_t i = a + blockIdx.x * blockDim.x + threadIdx.x; i < b; i += gridDim.x * blockDim.x) {
-        const uint32_t op = levelOps[i];
-        results[op] = *ops[op].occupied;
+    // The last workgroup of the level snapshots the counters: every other
+    // workgroup's counter updates precede its increment of done[level].  Only
+    // workgroups that own a chunk (block b < the level's chunk count) take
+    // part; with no chunk at all, workgroup 0 snapshots alone.
+    const uint32_t levelChunks = g1 - chunkCum[p0];
+    const uint32_t parts = levelChunks < gridDim.x ? levelChunks : gridDim.x;
+    if (parts == 0 ? blockIdx.x != 0 : blockIdx.x >= parts) return;
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = parts == 0 || atomicAdd(&done[level], 1u) == parts - 1;
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        for (uint32_t j = p0 + threadIdx.x; j < p1; j += kBatchBlock) {
+            const uint32_t op = levelOps[j];
+            results[op] = __hip_atomic_load(ops[op].occupied, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
-/// Host side: plans windows into levels and chunks and launches them.
+/// Host side: plans windows into levels and launches them.
 ///
 /// launch() is asynchronous; its occupancies (results()) are valid once the
 /// stream has passed the launch.  The executor's staging buffers are reused
@@ -196,15 +213,15 @@ class BatchExecutor {
         if ((e = growPinned(&hResults_, &capHResults_, count * sizeof(unsigned long long))) != hipSuccess) return e;
         if (moved) dropGraphs();
         if ((e = hipMemcpyAsync(dWin_, hStage_, staged_, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
-        const auto* win = static_cast<const unsigned char*>(dWin_);
+        auto* win = static_cast<unsigned char*>(dWin_);
         auto* results = static_cast<unsigned long long*>(dResults_);
         if (numLevels <= kMaxGraphLevels) {
             if (graphs_[numLevels] == nullptr && (e = capture(numLevels)) != hipSuccess) return e;
             if ((e = hipGraphLaunch(graphs_[numLevels], stream)) != hipSuccess) return e;
         } else {
+            const uint32_t grid = levelGrid();
             for (uint32_t l = 0; l < numLevels; ++l) {
-                batchLevelKernel<Config><<<levelGrid(), kBatchBlock, 0, stream>>>(win, l);
-                batchSnapshotKernel<uint32_t><<<4, 256, 0, stream>>>(win, l, results);
+                batchLevelKernel<Config><<<grid, kBatchBlock, 0, stream>>>(win, l, results);
             }
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
@@ -267,35 +284,28 @@ class BatchExecutor {
             cursor.assign(opStart_.begin(), opStart_.end() - 1);
             for (size_t i = 0; i < count; ++i) levelOps_[cursor[level_[i]]++] = static_cast<uint32_t>(i);
         }
-        chunks_.clear();
-        chunkStart_.assign(numLevels + 1, 0);
-        for (uint32_t l = 0; l < numLevels; ++l) {
-            chunkStart_[l] = static_cast<uint32_t>(chunks_.size());
-            for (uint32_t j = opStart_[l]; j < opStart_[l + 1]; ++j) {
-                const uint32_t op = levelOps_[j];
-                for (uint64_t off = 0; off < ops[op].n; off += kBatchChunk) {
-                    const uint64_t m = std::min<uint64_t>(kBatchChunk, ops[op].n - off);
-                    chunks_.push_back(BatchChunk{op, static_cast<uint32_t>(m), off});
-                }
-            }
+        // chunk counts in level order
+        chunkCum_.resize(count + 1);
+        chunkCum_[0] = 0;
+        for (size_t j = 0; j < count; ++j) {
+            const uint64_t n = ops[levelOps_[j]].n;
+            chunkCum_[j + 1] = chunkCum_[j] + static_cast<uint32_t>((n + kBatchChunk - 1) / kBatchChunk);
         }
-        chunkStart_[numLevels] = static_cast<uint32_t>(chunks_.size());
 
         WindowHeader h{};
         h.numOps = static_cast<uint32_t>(count);
-        h.numChunks = static_cast<uint32_t>(chunks_.size());
         h.numLevels = numLevels;
         size_t off = align64(sizeof(WindowHeader));
         h.opsOff = off;
         off = align64(off + count * sizeof(BatchOpDesc));
-        h.chunksOff = off;
-        off = align64(off + chunks_.size() * sizeof(BatchChunk));
         h.levelOpsOff = off;
         off = align64(off + count * sizeof(uint32_t));
-        h.chunkStartOff = off;
-        off = align64(off + (numLevels + 1) * sizeof(uint32_t));
+        h.chunkCumOff = off;
+        off = align64(off + (count + 1) * sizeof(uint32_t));
         h.opStartOff = off;
         off = align64(off + (numLevels + 1) * sizeof(uint32_t));
+        h.doneOff = off;
+        off = align64(off + numLevels * sizeof(uint32_t));
         staged_ = off;
         stageError_ = growPinned(&hStage_, &capStage_, staged_);
         if (stageError_ != hipSuccess) {
@@ -305,10 +315,10 @@ class BatchExecutor {
         auto* b = static_cast<unsigned char*>(hStage_);
         std::memcpy(b, &h, sizeof(h));
         std::memcpy(b + h.opsOff, ops, count * sizeof(BatchOpDesc));
-        std::memcpy(b + h.chunksOff, chunks_.data(), chunks_.size() * sizeof(BatchChunk));
         std::memcpy(b + h.levelOpsOff, levelOps_.data(), count * sizeof(uint32_t));
-        std::memcpy(b + h.chunkStartOff, chunkStart_.data(), (numLevels + 1) * sizeof(uint32_t));
+        std::memcpy(b + h.chunkCumOff, chunkCum_.data(), (count + 1) * sizeof(uint32_t));
         std::memcpy(b + h.opStartOff, opStart_.data(), (numLevels + 1) * sizeof(uint32_t));
+        std::memset(b + h.doneOff, 0, numLevels * sizeof(uint32_t));
         return numLevels;
     }
 
@@ -360,13 +370,12 @@ class BatchExecutor {
         hipError_t e;
         if (captureStream_ == nullptr && (e = hipStreamCreateWithFlags(&captureStream_, hipStreamNonBlocking)) != hipSuccess)
             return e;
-        const auto* win = static_cast<const unsigned char*>(dWin_);
+        auto* win = static_cast<unsigned char*>(dWin_);
         auto* results = static_cast<unsigned long long*>(dResults_);
         const uint32_t grid = levelGrid();
         if ((e = hipStreamBeginCapture(captureStream_, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
         for (uint32_t l = 0; l < numLevels; ++l) {
-            batchLevelKernel<Config><<<grid, kBatchBlock, 0, captureStream_>>>(win, l);
-            batchSnapshotKernel<uint32_t><<<4, 256, 0, captureStream_>>>(win, l, results);
+            batchLevelKernel<Config><<<grid, kBatchBlock, 0, captureStream_>>>(win, l, results);
         }
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(captureStream_, &graph);
@@ -394,8 +403,7 @@ class BatchExecutor {
     // planning scratch (host)
     std::unordered_map<const void*, uint32_t> seen_;
     std::vector<FilterLevelsPod> state_;
-    std::vector<uint32_t> level_, opStart_, levelOps_, chunkStart_, cursor_;
-    std::vector<BatchChunk> chunks_;
+    std::vector<uint32_t> level_, opStart_, levelOps_, chunkCum_, cursor_;
     // graphs by level count
     hipStream_t captureStream_ = nullptr;
     hipGraphExec_t graphs_[kMaxGraphLevels + 1] = {};
