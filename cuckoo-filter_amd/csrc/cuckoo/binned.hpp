@@ -49,11 +49,14 @@ inline constexpr int kBinBlock = 1024;
 template <typename Config, bool SortedSemantics>
 struct EntrySource {
     using Key = typename Config::KeyType;
+    using Raw = Key;
     const Key* keys;
     size_t numBuckets;
 
-    __device__ __forceinline__ uint64_t operator()(size_t i) const {
-        const auto c = Config::AltBucketPolicy::getCandidateBuckets(keys[i], numBuckets);
+    __device__ __forceinline__ Raw load(size_t i) const { return keys[i]; }
+    __device__ __forceinline__ uint64_t operator()(size_t i) const { return cook(keys[i]); }
+    __device__ __forceinline__ uint64_t cook(Key key) const {
+        const auto c = Config::AltBucketPolicy::getCandidateBuckets(key, numBuckets);
         uint64_t i1 = c.i1;
         if constexpr (SortedSemantics) {
             using P = std::conditional_t<Config::bitsPerTag <= 8, uint32_t, uint64_t>;
@@ -94,6 +97,9 @@ __global__ __launch_bounds__(256) void sliceStartKernel(const T* __restrict__ en
 /// Overflow entries a workgroup buffers in LDS before reserving their global
 /// space with a single atomic (more than this spill with one atomic per wave).
 inline constexpr uint32_t kOverflowLds = 2048;
+
+/// Entries each thread of binInsertKernel loads at once.
+inline constexpr uint32_t kBinDepth = 4;
 
 /// Dynamic LDS of binInsertKernel: the table slice, then the overflow buffer.
 inline constexpr size_t binOverflowOffset(size_t sliceBytes) { return (sliceBytes + 15) / 16 * 16; }
@@ -141,28 +147,41 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const uint64_t* __r
     uint32_t ok = 0;
     const int lane = __lane_id();
     const uint64_t ltMask = (1ull << lane) - 1;
-    uint64_t nextEntry = lo + threadIdx.x < hi ? entries[lo + threadIdx.x] : 0;
-    for (uint32_t e = lo + threadIdx.x; e < hi; e += kBinBlock) {
-        const uint64_t entry = nextEntry;
-        if (e + kBinBlock < hi) nextEntry = entries[e + kBinBlock];  // one entry ahead
-        const size_t local = static_cast<size_t>(entry >> 32) - b0;
-        const Tag fp = static_cast<Tag>(entry);
-        Word* bw = slice + local * WC;
-        const bool placed = Ops::tryInsertSnap(bw, detail::loadBucket<Word, WC>(bw), fp);
-        ok += placed;
-        const uint64_t fails = __ballot(!placed);
-        if (fails != 0) {
-            // one LDS atomic per wave reserves this wave's run in the workgroup buffer
-            const int leader = __ffsll(static_cast<long long>(fails)) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&ovCount, static_cast<uint32_t>(__popcll(fails)));
-            base = __shfl(base, leader, 64);
-            const uint32_t mine = base + static_cast<uint32_t>(__popcll(fails & ltMask));
-            if (!placed) {
-                if (mine < kOverflowLds) {
-                    ovBuf[mine] = entry;
-                } else {
-                    overflow[atomicAdd(overflowCount, 1ull)] = entry;  // buffer full (rare)
+    // kBinDepth entries per thread per step, loaded together (each load is
+    // coalesced across the wave), then inserted one after another
+    for (uint32_t e0 = lo + threadIdx.x; e0 < hi; e0 += kBinBlock * kBinDepth) {
+        uint64_t ent[kBinDepth];
+#pragma unroll
+        for (uint32_t k = 0; k < kBinDepth; ++k) {
+            const uint32_t e = e0 + k * kBinBlock;
+            ent[k] = e < hi ? entries[e] : 0;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kBinDepth; ++k) {
+            const bool valid = e0 + k * kBinBlock < hi;
+            const uint64_t entry = ent[k];
+            bool placed = true;
+            if (valid) {
+                const size_t local = static_cast<size_t>(entry >> 32) - b0;
+                const Tag fp = static_cast<Tag>(entry);
+                Word* bw = slice + local * WC;
+                placed = Ops::tryInsertSnap(bw, detail::loadBucket<Word, WC>(bw), fp);
+                ok += placed;
+            }
+            const uint64_t fails = __ballot(!placed);
+            if (fails != 0) {
+                // one LDS atomic per wave reserves this wave's run in the workgroup buffer
+                const int leader = __ffsll(static_cast<long long>(fails)) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&ovCount, static_cast<uint32_t>(__popcll(fails)));
+                base = __shfl(base, leader, 64);
+                const uint32_t mine = base + static_cast<uint32_t>(__popcll(fails & ltMask));
+                if (!placed) {
+                    if (mine < kOverflowLds) {
+                        ovBuf[mine] = entry;
+                    } else {
+                        overflow[atomicAdd(overflowCount, 1ull)] = entry;  // buffer full (rare)
+                    }
                 }
             }
         }

@@ -19,10 +19,13 @@ namespace cuckoo {
 template <typename Config>
 struct PackedTagSource {
     using P = PackedTagOf<Config>;
+    using Raw = typename Config::KeyType;
     const typename Config::KeyType* keys;
     size_t numBuckets;
-    __device__ __forceinline__ P operator()(size_t i) const {
-        const auto c = Config::AltBucketPolicy::getCandidateBuckets(keys[i], numBuckets);
+    __device__ __forceinline__ Raw load(size_t i) const { return keys[i]; }
+    __device__ __forceinline__ P operator()(size_t i) const { return cook(keys[i]); }
+    __device__ __forceinline__ P cook(Raw key) const {
+        const auto c = Config::AltBucketPolicy::getCandidateBuckets(key, numBuckets);
         return (static_cast<P>(c.i1) << Config::bitsPerTag) | static_cast<P>(c.fp);
     }
 };

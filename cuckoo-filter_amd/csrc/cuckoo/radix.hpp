@@ -15,7 +15,10 @@
 //    tile, then written out so that each digit's run is a contiguous,
 //    coalesced store.  The scatter is stable, as LSD needs.
 // The first pass reads its values through a source functor, so the values
-// (e.g. entries computed from keys) need not be materialised beforehand.
+// (e.g. entries computed from keys) need not be materialised beforehand; its
+// scatter runs as a resident grid that loads its next tile while it hashes,
+// ranks and writes the current one.  Later passes read arrays and scatter one
+// tile per workgroup.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,10 +42,15 @@ inline constexpr int kScanChunk = kScanBlock * kScanItems;
 static_assert(kTile % (64 * kWaves) == 0, "tile must split evenly into wave rows");
 static_assert(kRowsPerWave == kItems, "each lane keeps one value per row in registers");
 
-/// Reads element i of an array.
+/// Reads element i of an array.  Sources split a read into load() (the
+/// memory access) and cook() (any arithmetic on it), so that a kernel can
+/// issue the loads of its next tile before it needs the values.
 template <typename U>
 struct ArraySource {
+    using Raw = U;
     const U* data;
+    __device__ __forceinline__ Raw load(size_t i) const { return data[i]; }
+    __device__ __forceinline__ U cook(Raw r) const { return r; }
     __device__ __forceinline__ U operator()(size_t i) const { return data[i]; }
 };
 
@@ -84,10 +92,20 @@ __global__ __launch_bounds__(kBlock) void countKernel(Src src, size_t n, int shi
     for (int i = threadIdx.x; i < kRadix; i += kBlock) hist[i] = 0;
     __syncthreads();
     const size_t base = static_cast<size_t>(blockIdx.x) * kTile;
-#pragma unroll 4
+    // all loads first and unconditional (past-the-end lanes read element
+    // n - 1 and are not counted), then the arithmetic and the LDS counts
+    const size_t last = n - 1;
+    typename Src::Raw raw[kItems];
+#pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const size_t idx = base + static_cast<size_t>(k) * kBlock + threadIdx.x;
-        if (idx < n) atomicAdd(&hist[static_cast<uint32_t>(static_cast<U>(src(idx)) >> shift) & (kRadix - 1)], 1u);
+        raw[k] = src.load(idx < last ? idx : last);
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const size_t idx = base + static_cast<size_t>(k) * kBlock + threadIdx.x;
+        const uint32_t d = static_cast<uint32_t>(static_cast<U>(src.cook(raw[k])) >> shift) & (kRadix - 1);
+        if (idx < n) atomicAdd(&hist[d], 1u);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kRadix; i += kBlock) counts[static_cast<size_t>(i) * numTiles + blockIdx.x] = hist[i];
@@ -163,8 +181,126 @@ constexpr size_t scatterLdsBytes() {
     return static_cast<size_t>(kTile) * sizeof(U);
 }
 
+/// Pipelined, for sources that compute their values (such as hashing keys):
+/// a resident grid in which workgroup b handles tiles b, b + gridDim.x, ...
+/// and issues the loads of its next tile before the current tile is ranked,
+/// staged and written, so the loads overlap the arithmetic and LDS work.  It
+/// is held to 80 VGPRs so that three workgroups (the LDS limit) fit per CU.
 template <typename U, typename Src>
-__global__ __launch_bounds__(kBlock) void scatterKernel(Src src, U* __restrict__ out, size_t n, int shift,
+__global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, U* __restrict__ out, size_t n,
+                                                                         int shift,
+                                                                         const uint32_t* __restrict__ offsets,
+                                                                         uint32_t numTiles) {
+    using Raw = typename Src::Raw;
+    extern __shared__ __align__(16) unsigned char ldsRaw[];
+    U* stage = reinterpret_cast<U*>(ldsRaw);
+    __shared__ uint32_t cursor[kWaves][kRadix];
+    __shared__ uint32_t digitStart[kRadix];
+    __shared__ uint32_t globalBase[kRadix];
+
+    const int lane = __lane_id();
+    const int wave = threadIdx.x / 64;
+    const uint64_t ltMask = (1ull << lane) - 1;
+    static_assert(kRowsPerWave * kDigitBits <= 64, "digits of a lane must pack into 64 bits");
+
+    uint32_t t = blockIdx.x;
+    if (t >= numTiles) return;
+    // Loads are unconditional (past-the-end lanes read element n - 1 and are
+    // masked out later), so the compiler can count them: a wait for one
+    // value then does not wait for the loads issued after it.
+    const size_t last = n - 1;
+    Raw raw[kRowsPerWave];
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const size_t idx = static_cast<size_t>(t) * kTile + static_cast<size_t>(wave) * kRowsPerWave * 64 +
+                           static_cast<size_t>(r) * 64 + lane;
+        raw[r] = src.load(idx < last ? idx : last);
+    }
+    for (; t < numTiles; t += gridDim.x) {
+        const size_t tileBase = static_cast<size_t>(t) * kTile;
+        const size_t waveBase = tileBase + static_cast<size_t>(wave) * kRowsPerWave * 64;
+        const size_t tileCount = n - tileBase < static_cast<size_t>(kTile) ? n - tileBase : static_cast<size_t>(kTile);
+        for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&cursor[0][0])[i] = 0;
+        // this tile's values; then the next tile's loads go out
+        U vals[kRowsPerWave];
+#pragma unroll
+        for (int r = 0; r < kRowsPerWave; ++r) vals[r] = src.cook(raw[r]);
+        // this tile's digit bases, loaded before the prefetch so that waiting
+        // for them does not wait for the next tile's loads
+        const uint32_t myBase =
+            offsets[static_cast<size_t>(threadIdx.x < kRadix ? threadIdx.x : kRadix - 1) * numTiles + t];
+        {
+            // (past the last tile this reloads the current one; those values go unused)
+            const uint32_t tn = t + gridDim.x < numTiles ? t + gridDim.x : t;
+#pragma unroll
+            for (int r = 0; r < kRowsPerWave; ++r) {
+                const size_t idx = static_cast<size_t>(tn) * kTile +
+                                   static_cast<size_t>(wave) * kRowsPerWave * 64 + static_cast<size_t>(r) * 64 + lane;
+                raw[r] = src.load(idx < last ? idx : last);
+            }
+        }
+        uint64_t digits = 0;
+#pragma unroll
+        for (int r = 0; r < kRowsPerWave; ++r) {
+            digits |= static_cast<uint64_t>(static_cast<uint32_t>(vals[r] >> shift) & (kRadix - 1)) << (r * kDigitBits);
+        }
+        __syncthreads();  // cursor cleared
+        // 1. count digits per wave (ballot ranking, rolled)
+#pragma unroll 1
+        for (int r = 0; r < kRowsPerWave; ++r) {
+            const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
+            const bool valid = idx < n;
+            const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
+            const uint64_t peers = digitPeers(d, valid);
+            if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
+        }
+        __syncthreads();
+        // 2. digit start inside the tile, each wave's offset inside the digit, and the global base
+        uint32_t tileTotal = 0;
+        if (threadIdx.x < kRadix) {
+            const int d = threadIdx.x;
+            for (int w = 0; w < kWaves; ++w) tileTotal += cursor[w][d];
+        }
+        uint32_t total;
+        const uint32_t start = blockExclusiveScan<kBlock>(threadIdx.x < kRadix ? tileTotal : 0u, total);
+        if (threadIdx.x < kRadix) {
+            const int d = threadIdx.x;
+            digitStart[d] = start;
+            globalBase[d] = myBase;
+            uint32_t run = start;
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = cursor[w][d];
+                cursor[w][d] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        // 3. stage values digit-ordered (stable) in LDS
+#pragma unroll
+        for (int r = 0; r < kRowsPerWave; ++r) {
+            const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
+            const bool valid = idx < n;
+            const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
+            const uint64_t peers = digitPeers(d, valid);
+            const uint32_t pos = cursor[wave][d] + static_cast<uint32_t>(__popcll(peers & ltMask));
+            if (valid) stage[pos] = vals[r];
+            if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
+        }
+        __syncthreads();
+        // 4. contiguous runs per digit to their global place
+        for (uint32_t j = threadIdx.x; j < tileCount; j += kBlock) {
+            const U v = stage[j];
+            const uint32_t d = static_cast<uint32_t>(v >> shift) & (kRadix - 1);
+            out[globalBase[d] + (j - digitStart[d])] = v;
+        }
+        __syncthreads();  // stage, cursor and the bases are reused by the next tile
+    }
+}
+
+/// One tile per workgroup (the grid is the tile count): used for passes
+/// that read their values straight from an array.
+template <typename U, typename Src>
+__global__ __launch_bounds__(kBlock) void scatterTileKernel(Src src, U* __restrict__ out, size_t n, int shift,
                                                         const uint32_t* __restrict__ offsets, uint32_t numTiles) {
     extern __shared__ __align__(16) unsigned char ldsRaw[];
     U* stage = reinterpret_cast<U*>(ldsRaw);
@@ -246,6 +382,25 @@ __global__ __launch_bounds__(kBlock) void scatterKernel(Src src, U* __restrict__
     }
 }
 
+/// Grid of the pipelined scatter kernel: its resident workgroups on the
+/// current device (at most one per tile).
+template <typename U, typename Src>
+inline uint32_t scatterGrid(uint32_t numTiles) {
+    int dev = 0;
+    int cus = 0;
+    int perCu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, scatterKernel<U, Src>, kBlock,
+                                                     scatterLdsBytes<U>()) !=
+            hipSuccess ||
+        cus <= 0 || perCu <= 0) {
+        return numTiles;
+    }
+    const uint32_t resident = static_cast<uint32_t>(cus) * static_cast<uint32_t>(perCu);
+    return numTiles < resident ? numTiles : resident;
+}
+
 /// Scratch needed to partition n values: the two value buffers are the
 /// caller's; this is the count/scan space in bytes.
 inline size_t scratchBytes(size_t n) {
@@ -283,9 +438,10 @@ inline hipError_t sortBits(Src first, size_t n, int lo, int hi, U* a, U* b, void
         scanPartialsKernel<uint32_t><<<1, kScanBlock, 0, stream>>>(partial, parts);
         scanDownKernel<uint32_t><<<parts, kScanBlock, 0, stream>>>(counts, m, partial);
         if (firstPass) {
-            scatterKernel<U, Src><<<tiles, kBlock, scatterLdsBytes<U>(), stream>>>(first, dst, n, shift, counts, tiles);
+            scatterKernel<U, Src><<<scatterGrid<U, Src>(tiles), kBlock, scatterLdsBytes<U>(), stream>>>(
+                first, dst, n, shift, counts, tiles);
         } else {
-            scatterKernel<U, ArraySource<U>><<<tiles, kBlock, scatterLdsBytes<U>(), stream>>>(
+            scatterTileKernel<U, ArraySource<U>><<<tiles, kBlock, scatterLdsBytes<U>(), stream>>>(
                 ArraySource<U>{other}, dst, n, shift, counts, tiles);
         }
         firstPass = false;

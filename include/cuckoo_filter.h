@@ -238,7 +238,8 @@ typedef struct cf_server_stats {
 
 int cf_server_options_init(cf_server_options* o);
 int cf_server_create(const cf_server_options* o, cf_server** out);
-/* Finishes every submitted request, then stops the threads and frees. */
+/* Finishes every submitted request, then stops the threads and frees.  No
+ * other call on this server may run concurrently with it or follow it. */
 int cf_server_destroy(cf_server* s);
 /* Queues one request; *ticket names it for cf_server_wait / cf_server_poll.
  * The request's buffers must stay valid until it has completed. */

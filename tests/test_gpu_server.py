@@ -227,3 +227,23 @@ def test_server_ticket_api_and_errors(cf, oracle_mod):
         dk = cf.keys_to_device(big)
         dout = torch.zeros(big.size, dtype=torch.bool, device="cuda")
         assert srv.wait(srv.submit(f, cf.OP_LOOKUP, dk, dout))["status"] == 0
+
+
+def test_server_destroy_finishes_pending_requests(cf, oracle_mod):
+    import torch
+
+    O = oracle_mod
+    f = cf.CuckooFilter(1 << 18, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    keys = O.gen_keys(900, 200_000)
+    dk = cf.keys_to_device(keys)
+    outs = []
+    srv = cf.Server(max_window_requests=4)
+    for j in range(0, keys.size, 2000):
+        srv.submit(f, cf.OP_INSERT, dk[j:j + 2000])
+    for j in range(0, keys.size, 5000):
+        o = torch.zeros(5000, dtype=torch.bool, device="cuda")
+        srv.submit(f, cf.OP_LOOKUP, dk[j:j + 5000], o)
+        outs.append(o)
+    srv.close()  # returns once every queued request has run
+    assert f.occupiedSlots() == keys.size
+    assert all(bool(o.all().item()) for o in outs)
