@@ -92,9 +92,10 @@ struct Entry {
         if (*state == nullptr) *state = new cuckoo::BatchExecutor<Cfg>();
         return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->execute(ops, n, s, occupiedOut);
     }
-    static hipError_t batchLaunch(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t s) {
+    static hipError_t batchLaunch(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t s,
+                                  bool useGraph) {
         if (*state == nullptr) *state = new cuckoo::BatchExecutor<Cfg>();
-        return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->launch(ops, n, s);
+        return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->launch(ops, n, s, useGraph);
     }
     static const unsigned long long* batchResults(const void* state) {
         return static_cast<const cuckoo::BatchExecutor<Cfg>*>(state)->results();

@@ -18,7 +18,9 @@
 // from pinned memory.
 // The kernels find everything through the header, so a captured graph
 // depends only on the level count and stays valid for every later window
-// with that many levels; graphs are cached per level count.
+// with that many levels; graphs are cached per level count.  Callers whose
+// windows all differ (the serving loop) launch level by level instead, each
+// level with a grid sized to its own work.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -200,8 +202,11 @@ class BatchExecutor {
 
     /// Plans `ops`, uploads the window and launches it on `stream` without
     /// waiting; results()[i] holds op i's post-op occupancy once the stream
-    /// has passed this launch.
-    hipError_t launch(const BatchOpDesc* ops, size_t count, hipStream_t stream) {
+    /// has passed this launch.  With useGraph the levels run as the graph
+    /// cached for this level count (cheapest when windows of one shape
+    /// repeat); otherwise level by level, each with a grid no larger than its
+    /// chunk count (cheapest when every window differs, as in a server).
+    hipError_t launch(const BatchOpDesc* ops, size_t count, hipStream_t stream, bool useGraph = true) {
         if (count == 0) return hipSuccess;
         const uint32_t numLevels = plan(ops, count);
         if (stageError_ != hipSuccess) return stageError_;
@@ -215,13 +220,16 @@ class BatchExecutor {
         if ((e = hipMemcpyAsync(dWin_, hStage_, staged_, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
         auto* win = static_cast<unsigned char*>(dWin_);
         auto* results = static_cast<unsigned long long*>(dResults_);
-        if (numLevels <= kMaxGraphLevels) {
+        if (useGraph && numLevels <= kMaxGraphLevels) {
             if (graphs_[numLevels] == nullptr && (e = capture(numLevels)) != hipSuccess) return e;
             if ((e = hipGraphLaunch(graphs_[numLevels], stream)) != hipSuccess) return e;
         } else {
+            // level by level, each with a grid no larger than its chunk count
             const uint32_t grid = levelGrid();
             for (uint32_t l = 0; l < numLevels; ++l) {
-                batchLevelKernel<Config><<<grid, kBatchBlock, 0, stream>>>(win, l, results);
+                const uint32_t chunks = chunkCum_[opStart_[l + 1]] - chunkCum_[opStart_[l]];
+                const uint32_t g = chunks == 0 ? 1u : (chunks < grid ? chunks : grid);
+                batchLevelKernel<Config><<<g, kBatchBlock, 0, stream>>>(win, l, results);
             }
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }

@@ -148,7 +148,8 @@ int cf_server::launchWindow(Slot& sl) {
     hipError_t e = hipSuccess;
     if (keyBytes != 0) e = hipMemcpyAsync(d, h, keyBytes, hipMemcpyHostToDevice, stream);
     void** st = static_cast<void**>(executorFor(sl, sl.ops));
-    if (e == hipSuccess) e = sl.ops->batchLaunch(st, sl.descs.data(), sl.descs.size(), stream);
+    // windows differ from one to the next: level by level, each level's grid sized to its work
+    if (e == hipSuccess) e = sl.ops->batchLaunch(st, sl.descs.data(), sl.descs.size(), stream, false);
     sl.state = *st;
     if (e == hipSuccess && outBytes > keyBytes)
         e = hipMemcpyAsync(h + keyBytes, d + keyBytes, outBytes - keyBytes, hipMemcpyDeviceToHost, stream);
