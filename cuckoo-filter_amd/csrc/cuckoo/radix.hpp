@@ -7,8 +7,10 @@
 // the reference's full 64-bit sort (CuckooFilter.cuh:485-498) is not needed.
 //
 // Each pass is count -> scan -> scatter over tiles of kTile values:
-//  * count: per-tile digit histogram in LDS;
-//  * scan: exclusive scan of the digit-major (digit, tile) count matrix;
+//  * count: per-tile digit histogram in LDS, written as one contiguous row
+//    of a tile-major count matrix;
+//  * scan: the digit-major exclusive scan of that matrix, done over chunks of
+//    rows so that every access to it is a contiguous row;
 //  * scatter: a tile is split into 64-value rows, each wave owns a contiguous
 //    run of rows, a value's rank inside its row comes from eight ballots (one
 //    per digit bit); values are first placed digit-ordered in an LDS staging
@@ -108,7 +110,8 @@ __global__ __launch_bounds__(kBlock) void countKernel(Src src, size_t n, int shi
         if (idx < n) atomicAdd(&hist[d], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kRadix; i += kBlock) counts[static_cast<size_t>(i) * numTiles + blockIdx.x] = hist[i];
+    // tile-major: this tile's 256 counts are one contiguous row
+    for (int i = threadIdx.x; i < kRadix; i += kBlock) counts[static_cast<size_t>(blockIdx.x) * kRadix + i] = hist[i];
 }
 
 /// Sum of each kScanChunk-long chunk of `data`.  (The scan kernels are
@@ -160,6 +163,45 @@ __global__ __launch_bounds__(kScanBlock) void scanDownKernel(C* __restrict__ dat
     for (int k = 0; k < kScanItems; ++k) {
         if (base + k < m) data[base + k] = run;
         run += v[k];
+    }
+}
+
+/// The digit-ordered exclusive scan of a tile-major count matrix
+/// (counts[tile][digit]) runs over chunks of kChunkTiles rows, so that every
+/// access to the matrix is a contiguous row:
+///  1. chunkSumKernel: per chunk, the per-digit sums, stored digit-major
+///     (chunkSum[digit][chunk], a small array);
+///  2. the linear scan above over chunkSum: its exclusive prefix in
+///     digit-major order is each (digit, chunk)'s global start;
+///  3. chunkDownKernel: per chunk, each digit's running offset down the rows,
+///     written back in place.
+inline constexpr uint32_t kChunkTiles = 256;
+
+template <typename C = uint32_t>
+__global__ __launch_bounds__(kRadix) void chunkSumKernel(const C* __restrict__ counts, uint32_t numTiles,
+                                                         C* __restrict__ chunkSum, uint32_t numChunks) {
+    const uint32_t c = blockIdx.x;
+    const uint32_t t0 = c * kChunkTiles;
+    const uint32_t t1 = numTiles - t0 < kChunkTiles ? numTiles : t0 + kChunkTiles;
+    uint32_t sum = 0;
+#pragma unroll 8
+    for (uint32_t t = t0; t < t1; ++t) sum += counts[static_cast<size_t>(t) * kRadix + threadIdx.x];
+    chunkSum[static_cast<size_t>(threadIdx.x) * numChunks + c] = sum;
+}
+
+template <typename C = uint32_t>
+__global__ __launch_bounds__(kRadix) void chunkDownKernel(C* __restrict__ counts, uint32_t numTiles,
+                                                          const C* __restrict__ chunkStart, uint32_t numChunks) {
+    const uint32_t c = blockIdx.x;
+    const uint32_t t0 = c * kChunkTiles;
+    const uint32_t t1 = numTiles - t0 < kChunkTiles ? numTiles : t0 + kChunkTiles;
+    uint32_t run = chunkStart[static_cast<size_t>(threadIdx.x) * numChunks + c];
+#pragma unroll 8
+    for (uint32_t t = t0; t < t1; ++t) {
+        C* p = counts + static_cast<size_t>(t) * kRadix + threadIdx.x;
+        const uint32_t v = *p;
+        *p = run;
+        run += v;
     }
 }
 
@@ -228,7 +270,7 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, U* __restric
         // this tile's digit bases, loaded before the prefetch so that waiting
         // for them does not wait for the next tile's loads
         const uint32_t myBase =
-            offsets[static_cast<size_t>(threadIdx.x < kRadix ? threadIdx.x : kRadix - 1) * numTiles + t];
+            offsets[static_cast<size_t>(t) * kRadix + (threadIdx.x < kRadix ? threadIdx.x : kRadix - 1)];
         {
             // (past the last tile this reloads the current one; those values go unused)
             const uint32_t tn = t + gridDim.x < numTiles ? t + gridDim.x : t;
@@ -353,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void scatterTileKernel(Src src, U* __restri
     if (threadIdx.x < kRadix) {
         const int d = threadIdx.x;
         digitStart[d] = start;
-        globalBase[d] = offsets[static_cast<size_t>(d) * numTiles + blockIdx.x];
+        globalBase[d] = offsets[static_cast<size_t>(blockIdx.x) * kRadix + d];
         uint32_t run = start;
         for (int w = 0; w < kWaves; ++w) {
             const uint32_t c = cursor[w][d];
@@ -406,8 +448,10 @@ inline uint32_t scatterGrid(uint32_t numTiles) {
 inline size_t scratchBytes(size_t n) {
     const size_t tiles = (n + kTile - 1) / kTile;
     const size_t m = tiles * kRadix;
-    const size_t parts = (m + kScanChunk - 1) / kScanChunk;
-    return (m + parts) * sizeof(uint32_t) + 256;
+    const size_t chunks = (tiles + kChunkTiles - 1) / kChunkTiles;
+    const size_t cm = chunks * kRadix;
+    const size_t parts = (cm + kScanChunk - 1) / kScanChunk;
+    return (m + cm + parts) * sizeof(uint32_t) + 256;
 }
 
 /// Sorts n values by bits [lo, hi), 8 bits per pass.  The first pass reads
@@ -421,9 +465,12 @@ inline hipError_t sortBits(Src first, size_t n, int lo, int hi, U* a, U* b, void
     if (n == 0 || hi <= lo) return hipSuccess;
     const uint32_t tiles = static_cast<uint32_t>((n + kTile - 1) / kTile);
     const size_t m = static_cast<size_t>(tiles) * kRadix;
-    const uint32_t parts = static_cast<uint32_t>((m + kScanChunk - 1) / kScanChunk);
-    uint32_t* counts = static_cast<uint32_t*>(scratch);
-    uint32_t* partial = counts + m;
+    const uint32_t chunks = (tiles + kChunkTiles - 1) / kChunkTiles;
+    const size_t cm = static_cast<size_t>(chunks) * kRadix;
+    const uint32_t parts = static_cast<uint32_t>((cm + kScanChunk - 1) / kScanChunk);
+    uint32_t* counts = static_cast<uint32_t*>(scratch);  // [tile][digit]
+    uint32_t* chunkSum = counts + m;                      // [digit][chunk]
+    uint32_t* partial = chunkSum + cm;
     U* dst = a;
     U* other = b;
     bool firstPass = true;
@@ -434,9 +481,11 @@ inline hipError_t sortBits(Src first, size_t n, int lo, int hi, U* a, U* b, void
             countKernel<U, ArraySource<U>><<<tiles, kBlock, 0, stream>>>(ArraySource<U>{other}, n, shift, counts,
                                                                           tiles);
         }
-        scanReduceKernel<uint32_t><<<parts, kScanBlock, 0, stream>>>(counts, m, partial);
+        chunkSumKernel<uint32_t><<<chunks, kRadix, 0, stream>>>(counts, tiles, chunkSum, chunks);
+        scanReduceKernel<uint32_t><<<parts, kScanBlock, 0, stream>>>(chunkSum, cm, partial);
         scanPartialsKernel<uint32_t><<<1, kScanBlock, 0, stream>>>(partial, parts);
-        scanDownKernel<uint32_t><<<parts, kScanBlock, 0, stream>>>(counts, m, partial);
+        scanDownKernel<uint32_t><<<parts, kScanBlock, 0, stream>>>(chunkSum, cm, partial);
+        chunkDownKernel<uint32_t><<<chunks, kRadix, 0, stream>>>(counts, tiles, chunkSum, chunks);
         if (firstPass) {
             scatterKernel<U, Src><<<scatterGrid<U, Src>(tiles), kBlock, scatterLdsBytes<U>(), stream>>>(
                 first, dst, n, shift, counts, tiles);
