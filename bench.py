@@ -291,6 +291,7 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
     target = np.where(rng.random(n_requests) < 0.5, 0, 1 + rng.integers(0, n_tenants, n_requests))
     clipped = 0
     total_keys = 0
+    origin = []  # (source tensor, start, n) of each request's keys
     for j in range(n_requests):
         i, kind, n = int(target[j]), int(kinds[j]), int(sizes[j])
         out = 0
@@ -298,6 +299,7 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
             pool = lpools[i]
             start = int(rng.integers(0, pool.numel() - n + 1))
             keys = pool.data_ptr() + 8 * start
+            origin.append((pool, start, n))
             if ring_at + n > ring.numel():
                 ring_at = 0
             out = ring.data_ptr() + ring_at
@@ -307,10 +309,12 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
             clipped += n - m
             n = m
             keys = pools[i].data_ptr() + 8 * hi[i]
+            origin.append((pools[i], hi[i], n))
             hi[i] += n
         else:  # delete the oldest present keys
             n = min(n, hi[i] - lo[i])
             keys = pools[i].data_ptr() + 8 * lo[i]
+            origin.append((pools[i], lo[i], n))
             lo[i] += n
         reqs[j] = (filters[i]._h.value, kind, 0, keys, n, out)
         total_keys += n
@@ -343,6 +347,34 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
             "max_queue": st["max_queue"],
         }
 
+    def host_burst(srv, count=2000):
+        """The first `count` requests with keys and results in pageable host
+        memory, replayed as one burst from the same starting states: the rate
+        includes the staging copies and PCIe in both directions."""
+        sub = reqs[:count].copy()
+        keep = []
+        for j in range(count):
+            pool, start, n = origin[j]
+            if n == 0:
+                continue
+            hk = np.ascontiguousarray(pool[start:start + n].cpu().numpy().view(np.uint64))
+            keep.append(hk)
+            sub["flags"][j] = cf.CF_REQUEST_HOST_MEMORY
+            sub["keys"][j] = hk.ctypes.data
+            if int(sub["op"][j]) == 1:
+                o = np.zeros(n, dtype=bool)
+                keep.append(o)
+                sub["output"][j] = o.ctypes.data
+            else:
+                sub["output"][j] = 0
+        keys_total = int(sub["n"].sum())
+        restore()
+        res = srv.replay(sub, np.zeros(count))
+        assert (res["status"] == 0).all(), "host-memory request failed"
+        span = (int(res["done_ns"].max()) - int(res["arrival_ns"].min())) / 1e9
+        return {"requests": count, "keys": keys_total, "mkeys_s": round(keys_total / span / 1e6, 1),
+                "gb_s_over_pcie": round(keys_total * 8 / span / 1e9, 2)}
+
     out = {
         "workload": ("canonical 31-mers of a 2^28-base sequence (10% duplicated regions) in a 2^30-slot filter "
                      "+ 64 tenants of 2^20..2^24 slots; requests log-uniform 1..2^18 keys, 80/15/5 "
@@ -357,6 +389,7 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
         run(srv, np.zeros(n_requests))  # warm-up: plans, graphs, arenas
         sat = run(srv, np.zeros(n_requests))
         out["saturated"] = sat
+        out["host_memory_burst"] = host_burst(srv)
         peak = sat["mkeys_s"] * 1e6
         mean_keys = total_keys / n_requests
         for frac in loads:

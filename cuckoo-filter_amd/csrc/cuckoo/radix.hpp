@@ -237,8 +237,7 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, U* __restric
     extern __shared__ __align__(16) unsigned char ldsRaw[];
     U* stage = reinterpret_cast<U*>(ldsRaw);
     __shared__ uint32_t cursor[kWaves][kRadix];
-    __shared__ uint32_t digitStart[kRadix];
-    __shared__ uint32_t globalBase[kRadix];
+    __shared__ uint32_t runShift[kRadix];  // global start of the digit's run minus its start in the tile
 
     const int lane = __lane_id();
     const int wave = threadIdx.x / 64;
@@ -307,8 +306,7 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, U* __restric
         const uint32_t start = blockExclusiveScan<kBlock>(threadIdx.x < kRadix ? tileTotal : 0u, total);
         if (threadIdx.x < kRadix) {
             const int d = threadIdx.x;
-            digitStart[d] = start;
-            globalBase[d] = myBase;
+            runShift[d] = myBase - start;
             uint32_t run = start;
             for (int w = 0; w < kWaves; ++w) {
                 const uint32_t c = cursor[w][d];
@@ -333,7 +331,7 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, U* __restric
         for (uint32_t j = threadIdx.x; j < tileCount; j += kBlock) {
             const U v = stage[j];
             const uint32_t d = static_cast<uint32_t>(v >> shift) & (kRadix - 1);
-            out[globalBase[d] + (j - digitStart[d])] = v;
+            out[static_cast<uint32_t>(runShift[d] + j)] = v;  // exact: the sum is below n < 2^32
         }
         __syncthreads();  // stage, cursor and the bases are reused by the next tile
     }
@@ -347,8 +345,7 @@ __global__ __launch_bounds__(kBlock) void scatterTileKernel(Src src, U* __restri
     extern __shared__ __align__(16) unsigned char ldsRaw[];
     U* stage = reinterpret_cast<U*>(ldsRaw);
     __shared__ uint32_t cursor[kWaves][kRadix];
-    __shared__ uint32_t digitStart[kRadix];
-    __shared__ uint32_t globalBase[kRadix];
+    __shared__ uint32_t runShift[kRadix];  // global start of the digit's run minus its start in the tile
 
     const int lane = __lane_id();
     const int wave = threadIdx.x / 64;
@@ -394,8 +391,7 @@ __global__ __launch_bounds__(kBlock) void scatterTileKernel(Src src, U* __restri
     const uint32_t start = blockExclusiveScan<kBlock>(threadIdx.x < kRadix ? tileTotal : 0u, total);
     if (threadIdx.x < kRadix) {
         const int d = threadIdx.x;
-        digitStart[d] = start;
-        globalBase[d] = offsets[static_cast<size_t>(blockIdx.x) * kRadix + d];
+        runShift[d] = offsets[static_cast<size_t>(blockIdx.x) * kRadix + d] - start;
         uint32_t run = start;
         for (int w = 0; w < kWaves; ++w) {
             const uint32_t c = cursor[w][d];
@@ -420,7 +416,7 @@ __global__ __launch_bounds__(kBlock) void scatterTileKernel(Src src, U* __restri
     for (uint32_t j = threadIdx.x; j < tileCount; j += kBlock) {
         const U v = stage[j];
         const uint32_t d = static_cast<uint32_t>(v >> shift) & (kRadix - 1);
-        out[globalBase[d] + (j - digitStart[d])] = v;
+        out[static_cast<uint32_t>(runShift[d] + j)] = v;  // exact: the sum is below n < 2^32
     }
 }
 
