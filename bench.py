@@ -131,100 +131,43 @@ def cpu_baseline(n_ins, n_look, n_del, budget_s=25.0):
 
 def batched_section(cf, torch, workloads, dev, n_filters=256, n_batches=8192, seed=7):
     """Batched many-filter workload: n_filters filters with capacities
-    log-uniform in 2^14..2^26 slots (16-bit tags, bucket 16, BFS), each filled
-    to a uniform-random load in 0.3..0.95, driven by n_batches batches of
-    log-uniform 32..2^20 keys, ops 40% insert / 50% lookup (50% positive) /
-    10% delete, target filter uniform.  Inserts are clipped so that no filter
-    passes 95% load.  The same window runs as plain per-batch calls and as one
-    cf_batch_execute, each from the same starting filter states."""
-    import numpy as np
+    log-uniform in 2^14..2^26 slots, each filled to a uniform-random load in
+    0.3..0.95, driven by n_batches batches of log-uniform 32..2^20 keys, ops
+    40% insert / 50% lookup (50% positive) / 10% delete, target filter
+    uniform.  Inserts are clipped so that no filter passes 95% load.  The same
+    window runs as plain per-batch calls and as one cf_batch_execute, each
+    from the same starting filter states, first with 16-bit tags, bucket 16,
+    u64 words and BFS, then with the configuration the autotuner picks for
+    this op mix under the same false-positive budget."""
+    import autotune
 
-    rng = np.random.default_rng(seed)
-    cfg = cf.make_config(key_bytes=8, bits_per_tag=16, bucket_size=16, eviction="bfs")
-    caps = (2 ** rng.uniform(14, 26, n_filters)).astype(np.int64)
-    base, fill_n, counts = [], [], []
-    for i in range(n_filters):
-        f = cf.CuckooFilter(int(caps[i]), config=cfg)
-        n = int(f.capacity() * rng.uniform(0.3, 0.95))
-        f.insertMany(workloads.gen_keys_torch(1000 + i, n, device=dev))
-        base.append(f)
-        fill_n.append(n)
-        counts.append(f.occupiedSlots())
-    del_cursor = [0] * n_filters
-    ins_cursor = [0] * n_filters
-    ops_spec = []
-    total_keys = 0
-    for j in range(n_batches):
-        i = int(rng.integers(n_filters))
-        m = int(2 ** rng.uniform(5, 20))
-        kind = int(rng.choice(3, p=[0.4, 0.5, 0.1]))
-        if kind == 0:
-            m = max(0, min(m, int(0.95 * base[i].capacity()) - counts[i]))
-            keys = workloads.gen_keys_torch(3000 + i, m, start=ins_cursor[i], device=dev)
-            ins_cursor[i] += m
-            counts[i] += m
-        elif kind == 1:
-            live = fill_n[i] - del_cursor[i]
-            pos = min(m // 2, live)
-            start = del_cursor[i] + int(rng.integers(0, max(1, live - pos + 1)))
-            keys = torch.cat([workloads.gen_keys_torch(1000 + i, pos, start=start, device=dev),
-                              workloads.gen_keys_torch(5000 + j, m - pos, device=dev)])
-        else:
-            m = min(m, fill_n[i] - del_cursor[i])
-            keys = workloads.gen_keys_torch(1000 + i, m, start=del_cursor[i], device=dev)
-            del_cursor[i] += m
-            counts[i] -= m
-        out = torch.empty(max(m, 1), dtype=torch.bool, device=dev) if kind else None
-        ops_spec.append((i, kind, keys, out))
-        total_keys += m
-    plain = [cf.CuckooFilter(int(caps[i]), config=cfg) for i in range(n_filters)]
-    batched = [cf.CuckooFilter(int(caps[i]), config=cfg) for i in range(n_filters)]
-
-    def restore(fs):
-        for f, b in zip(fs, base):
-            f.copyFrom(b)
-        torch.cuda.synchronize()
-
-    def run_plain():
-        occ = []
-        for i, kind, keys, out in ops_spec:
-            f = plain[i]
-            if kind == 0:
-                occ.append(f.insertMany(keys))
-            elif kind == 1:
-                f.containsMany(keys, out=out)
-                occ.append(0)
-            else:
-                occ.append(f.deleteMany(keys, out=out))
-        return occ
-
-    b = cf.Batch()
-    arr = b.make_ops([(batched[i], kind, keys, out) for i, kind, keys, out in ops_spec])
-    restore(batched)
-    b.execute(arr)  # first window: plans and captures the graph
-    restore(plain)
-    t0 = time.perf_counter()
-    occ_p = run_plain()
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
-    restore(batched)
-    t0 = time.perf_counter()
-    occ_b = b.execute(arr)
-    torch.cuda.synchronize()
-    t_batched = time.perf_counter() - t0
-    same = all(int(occ_b[k]) == occ_p[k] for k in range(len(ops_spec)) if ops_spec[k][1] != 1)
-    b.close()
-    return {
-        "workload": (f"{n_filters} filters, capacities log-uniform 2^14..2^26 slots, fill U(0.3, 0.95); "
-                     f"{n_batches} batches log-uniform 32..2^20 keys, 40/50/10 insert/lookup/delete"),
-        "keys": total_keys,
-        "plain_mkeys_s": round(total_keys / t_plain / 1e6, 1),
-        "batched_mkeys_s": round(total_keys / t_batched / 1e6, 1),
-        "speedup": round(t_plain / t_batched, 2),
-        "plain_ms": round(t_plain * 1e3, 2),
-        "batched_ms": round(t_batched * 1e3, 2),
-        "occupancies_match_plain": bool(same),
+    default = dict(key_bytes=8, policy="xor", bits_per_tag=16, bucket_size=16, word_bytes=8, eviction="bfs",
+                   block_size=256)
+    out = autotune.batched_trial(default, n_filters, n_batches, seed, device=dev)
+    # stage 1 prunes on a scaled-down copy of the workload (64 filters, 1024
+    # batches); stage 2 reruns the shortlist and the default on the full size
+    load = 0.95
+    budget = autotune.fpr_closed_form(16, 16, load)
+    best, results, confirmed = autotune.choose_batched(budget, load=load, n_filters=64, n_batches=1024,
+                                                       baseline=default, confirm=(n_filters, n_batches))
+    tuned = best if best is not None else default
+    axes = ("bits_per_tag", "bucket_size", "word_bytes", "eviction")
+    out["autotune"] = {
+        "trial": ("stage 1: 64 filters, 1024 batches of the same distributions, every candidate meeting the "
+                  "default's FPR at 0.95; stage 2: the 4 fastest and the default on the full window (another "
+                  "seed, best of 3), the default kept unless beaten by 3%"),
+        "fpr_budget": budget,
+        "points": len(results),
+        "chosen": {k: tuned[k] for k in axes},
+        "stage1_top": [{**{k: kw[k] for k in axes}, "batched_mkeys_s": r["batched_mkeys_s"]}
+                       for kw, r in sorted(results, key=lambda x: -x[1]["batched_mkeys_s"])[:5]],
+        "stage2": [{**{k: kw[k] for k in axes}, "batched_mkeys_s": r["batched_mkeys_s"]} for kw, r in confirmed],
     }
+    if tuned != default:
+        t = autotune.batched_trial(tuned, n_filters, n_batches, seed, device=dev)
+        out["autotuned"] = {k: t[k] for k in ("plain_mkeys_s", "batched_mkeys_s", "speedup",
+                                              "occupancies_match_plain")}
+    return out
 
 
 def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(0.5, 0.7, 1.0)):

@@ -1,0 +1,344 @@
+"""Autotuner: measures filter configurations on the GPU and picks one.
+
+A candidate is (bits_per_tag, bucket_size, word_bytes, eviction, block_size)
+for a fixed key width and alternate-bucket policy; only combinations the
+library builds are considered (the reference's static_asserts,
+CuckooFilter.cuh:67-71, 147-152, 226-236).  For a workload -- table slots,
+target load, op mix, false-positive budget -- every candidate is measured
+through the C ABI on the workload's own shape:
+
+  * a filter of the workload's capacity is filled to the target load with
+    insertMany (timed with HIP events; a second, untimed fill with the
+    eviction-chain histogram on gives the chain statistics);
+  * containsMany over as many keys, half inserted and half never inserted
+    (timed; the never-inserted half gives the measured FPR);
+  * deleteMany of a quarter of the inserted keys (timed).
+
+Lookup bytes per key are algorithmic: key + 1-byte result + one bucket, plus
+a second bucket for every negative and for every positive whose tag does not
+sit in its primary bucket (counted on the exported table image).
+
+choose() returns the candidate with the highest op-mix-weighted rate among
+those that placed every key and whose measured FPR and closed-form FPR
+1 - (1 - 1/(2^f - 1))^(2*b*alpha) both stay within the budget.
+
+    python cuckoo-filter_amd/autotune.py --slots 16777216 --load 0.98 > sweep.jsonl
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+if _HERE not in sys.path:
+    sys.path.insert(0, _HERE)
+
+import cuckoo_amd as cf  # noqa: E402
+import workloads  # noqa: E402
+
+TAG_BYTES = {8: 1, 16: 2, 32: 4}
+
+
+def fpr_closed_form(bits_per_tag: int, bucket_size: int, load: float) -> float:
+    """Chance that an absent key matches some tag of its two buckets."""
+    return 1.0 - (1.0 - 1.0 / (2 ** bits_per_tag - 1)) ** (2 * bucket_size * load)
+
+
+def candidates(key_bytes=8, policy="xor", bits=(8, 16, 32), buckets=(4, 8, 16, 32), words=(4, 8),
+               evictions=("bfs", "dfs"), blocks=(64, 128, 256, 512)):
+    """Every combination the library builds, as make_config keyword dicts."""
+    out = []
+    for b, bs, w, ev, blk in itertools.product(bits, buckets, words, evictions, blocks):
+        if TAG_BYTES[b] > w or bs % (w // TAG_BYTES[b]):
+            continue
+        kw = dict(key_bytes=key_bytes, policy=policy, bits_per_tag=b, bucket_size=bs, word_bytes=w,
+                  eviction=ev, block_size=blk)
+        if cf.config_supported(cf.make_config(**kw)):
+            out.append(kw)
+    return out
+
+
+def _events(torch):
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def _timed(torch, fn):
+    a, b = _events(torch)
+    a.record()
+    r = fn()
+    b.record()
+    torch.cuda.synchronize()
+    return r, a.elapsed_time(b) / 1e3
+
+
+def _primary_share(f, keys_dev, sample):
+    """Share of inserted keys (of a sample) whose tag sits in the primary bucket."""
+    c = f.config
+    k = keys_dev[:sample]
+    i1, _, fp = f.computeCandidates(k)
+    i1 = cf.to_numpy_u(i1, np.uint64)
+    fp = cf.to_numpy_u(fp, np.uint32)
+    img = f.exportTable()
+    tb = TAG_BYTES[c.bits_per_tag]
+    tags = img.view({1: np.uint8, 2: np.uint16, 4: np.uint32}[tb]).reshape(-1, c.bucket_size)
+    hit = (tags[i1.astype(np.int64)] == fp[:, None].astype(tags.dtype)).any(axis=1)
+    return float(hit.mean())
+
+
+def measure(kw, slots, load, seed=1, device="cuda", primary_sample=1 << 16):
+    """One candidate on one workload shape; returns a result dict."""
+    import torch
+
+    f = cf.CuckooFilter(slots, config=cf.make_config(**kw))
+    try:
+        cap = f.capacity()
+        n = int(cap * load)
+        key_bytes = kw["key_bytes"]
+        keys = workloads.gen_keys_torch(seed, 2 * n, device=device, key_bytes=key_bytes)
+        pos, neg = keys[:n], keys[n:]
+        query = torch.cat([pos[: n // 2], neg[: n - n // 2]])
+        out = torch.empty(query.numel(), dtype=torch.bool, device=device)
+        dels = pos[: n // 4]
+        # chain statistics from an untimed fill
+        f.setEvictionHistogram(True)
+        f.insertMany(pos)
+        hist = f.evictionHistogram()
+        f.setEvictionHistogram(False)
+        f.clear()
+        occ, t_ins = _timed(torch, lambda: f.insertMany(pos))
+        _, t_look = _timed(torch, lambda: f.containsMany(query, out=out))
+        fps = int(out[n // 2:].sum().item())
+        share = _primary_share(f, pos, min(primary_sample, n))
+        _, t_del = _timed(torch, lambda: f.deleteMany(dels))
+        bucket_bytes = kw["bucket_size"] * TAG_BYTES[kw["bits_per_tag"]]
+        # positives: one bucket, a second when the tag is not in the primary; negatives: two
+        lookup_bytes = key_bytes + 1 + bucket_bytes * (0.5 * (1 + (1 - share)) + 0.5 * 2)
+        chains = np.arange(hist.size - 1)
+        placed_by_eviction = int(hist[1:-1].sum())
+        return {
+            **kw,
+            "slots": cap,
+            "load": round(occ / cap, 6),
+            "keys": n,
+            "failed": n - occ,
+            "insert_mkeys_s": round(n / t_ins / 1e6, 1),
+            "lookup_mkeys_s": round(query.numel() / t_look / 1e6, 1),
+            "delete_mkeys_s": round(dels.numel() / t_del / 1e6, 1),
+            "lookup_bucket_gbs": round(lookup_bytes * query.numel() / t_look / 1e9, 1),
+            "fpr": fps / (n - n // 2),
+            "fpr_closed_form": fpr_closed_form(kw["bits_per_tag"], kw["bucket_size"], occ / cap),
+            "evicting_inserts": placed_by_eviction,
+            "mean_chain": round(float((hist[1:-1] * chains[1:]).sum() / max(placed_by_eviction, 1)), 3),
+            "max_chain": int(chains[1:][hist[1:-1] > 0].max()) if placed_by_eviction else 0,
+            "exhausted": int(hist[-1]),
+            "primary_share": round(share, 4),
+            # u32 words, two per bucket: the reference's 128-bit lookup reads past
+            # the bucket here (CuckooFilter.cuh:272-277, 332-342); this build's
+            # lookup is the semantic one (some slot equals the tag)
+            "reference_lookup_undefined": kw["word_bytes"] == 4 and bucket_bytes == 8,
+        }
+    finally:
+        f.close()
+
+
+def score(r, mix=(0.4, 0.5, 0.1)):
+    """Keys per second of the op mix (insert, lookup, delete) at the measured rates."""
+    ins, look, dele = mix
+    t = ins / r["insert_mkeys_s"] + look / r["lookup_mkeys_s"] + dele / r["delete_mkeys_s"]
+    return 1.0 / t
+
+
+def choose(results, fpr_budget, mix=(0.4, 0.5, 0.1)):
+    """The best admissible result, or None.  Admissible: no failed insert, and
+    both the measured FPR (with a 3-sigma allowance for the sample) and the
+    closed form within the budget."""
+    ok = []
+    for r in results:
+        m = r["keys"] - r["keys"] // 2
+        sigma = np.sqrt(max(r["fpr"] * (1 - r["fpr"]), 1e-12) / max(m, 1))
+        if r["failed"] == 0 and r["fpr"] - 3 * sigma <= fpr_budget and r["fpr_closed_form"] <= fpr_budget:
+            ok.append(r)
+    return max(ok, key=lambda r: score(r, mix)) if ok else None
+
+
+def batched_trial(kw, n_filters=256, n_batches=8192, seed=7, device="cuda", mix=(0.4, 0.5, 0.1),
+                  cap_log2=(14, 26), fill=(0.3, 0.95), with_plain=True, repeats=1):
+    """The batched many-filter workload for configuration `kw`: n_filters
+    filters with capacities log-uniform in 2^cap_log2 slots, each filled to a
+    uniform-random load in `fill`, driven by n_batches batches of log-uniform
+    32..2^20 keys with ops drawn by `mix` (insert / lookup with 50% positives /
+    delete), target filter uniform; inserts are clipped so that no filter
+    passes 95% load.  The window runs as one cf_batch_execute (after one
+    untimed window that plans and captures the graphs) and, with with_plain,
+    as plain per-batch calls, each from the same starting states.  The batched
+    window is timed `repeats` times, each from the starting states, and the
+    fastest run counts."""
+    import time
+
+    import torch
+
+    if kw.get("key_bytes", 8) != 8:
+        raise ValueError("the batched trial generates 8-byte keys")
+    cfg = cf.make_config(**kw)
+    rng = np.random.default_rng(seed)
+    caps = (2 ** rng.uniform(cap_log2[0], cap_log2[1], n_filters)).astype(np.int64)
+    base, fill_n, counts = [], [], []
+    for i in range(n_filters):
+        f = cf.CuckooFilter(int(caps[i]), config=cfg)
+        n = int(f.capacity() * rng.uniform(fill[0], fill[1]))
+        f.insertMany(workloads.gen_keys_torch(1000 + i, n, device=device))
+        base.append(f)
+        fill_n.append(n)
+        counts.append(f.occupiedSlots())
+    del_cursor = [0] * n_filters
+    ins_cursor = [0] * n_filters
+    ops_spec = []
+    total_keys = 0
+    for j in range(n_batches):
+        i = int(rng.integers(n_filters))
+        m = int(2 ** rng.uniform(5, 20))
+        kind = int(rng.choice(3, p=list(mix)))
+        if kind == 0:
+            m = max(0, min(m, int(0.95 * base[i].capacity()) - counts[i]))
+            keys = workloads.gen_keys_torch(3000 + i, m, start=ins_cursor[i], device=device)
+            ins_cursor[i] += m
+            counts[i] += m
+        elif kind == 1:
+            live = fill_n[i] - del_cursor[i]
+            pos = min(m // 2, live)
+            start = del_cursor[i] + int(rng.integers(0, max(1, live - pos + 1)))
+            keys = torch.cat([workloads.gen_keys_torch(1000 + i, pos, start=start, device=device),
+                              workloads.gen_keys_torch(5000 + j, m - pos, device=device)])
+        else:
+            m = min(m, fill_n[i] - del_cursor[i])
+            keys = workloads.gen_keys_torch(1000 + i, m, start=del_cursor[i], device=device)
+            del_cursor[i] += m
+            counts[i] -= m
+        out = torch.empty(max(m, 1), dtype=torch.bool, device=device) if kind else None
+        ops_spec.append((i, kind, keys, out))
+        total_keys += m
+    plain = [cf.CuckooFilter(int(caps[i]), config=cfg) for i in range(n_filters)] if with_plain else []
+    batched = [cf.CuckooFilter(int(caps[i]), config=cfg) for i in range(n_filters)]
+
+    def restore(fs):
+        for f, b in zip(fs, base):
+            f.copyFrom(b)
+        torch.cuda.synchronize()
+
+    def run_plain():
+        occ = []
+        for i, kind, keys, out in ops_spec:
+            f = plain[i]
+            if kind == 0:
+                occ.append(f.insertMany(keys))
+            elif kind == 1:
+                f.containsMany(keys, out=out)
+                occ.append(0)
+            else:
+                occ.append(f.deleteMany(keys, out=out))
+        return occ
+
+    b = cf.Batch()
+    arr = b.make_ops([(batched[i], kind, keys, out) for i, kind, keys, out in ops_spec])
+    restore(batched)
+    b.execute(arr)  # first window: plans and captures the graph
+    if with_plain:
+        restore(plain)
+        t0 = time.perf_counter()
+        occ_p = run_plain()
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+    t_batched = float("inf")
+    for _ in range(max(1, repeats)):
+        restore(batched)
+        t0 = time.perf_counter()
+        occ_b = b.execute(arr)
+        torch.cuda.synchronize()
+        t_batched = min(t_batched, time.perf_counter() - t0)
+    same = with_plain and all(int(occ_b[k]) == occ_p[k] for k in range(len(ops_spec)) if ops_spec[k][1] != 1)
+    b.close()
+    for f in base + plain + batched:
+        f.close()
+    if not with_plain:
+        return {"keys": total_keys, "batched_mkeys_s": round(total_keys / t_batched / 1e6, 1),
+                "batched_ms": round(t_batched * 1e3, 2)}
+    return {
+        "workload": (f"{n_filters} filters, capacities log-uniform 2^{cap_log2[0]}..2^{cap_log2[1]} slots, fill U{fill}; "
+                     f"{n_batches} batches log-uniform 32..2^20 keys, insert/lookup/delete mix {mix}"),
+        "keys": total_keys,
+        "plain_mkeys_s": round(total_keys / t_plain / 1e6, 1),
+        "batched_mkeys_s": round(total_keys / t_batched / 1e6, 1),
+        "speedup": round(t_plain / t_batched, 2),
+        "plain_ms": round(t_plain * 1e3, 2),
+        "batched_ms": round(t_batched * 1e3, 2),
+        "occupancies_match_plain": bool(same),
+    }
+
+
+def choose_batched(fpr_budget, load=0.95, n_filters=64, n_batches=1024, seed=11, mix=(0.4, 0.5, 0.1), cands=None,
+                   baseline=None, shortlist=4, confirm=(256, 8192), repeats=3, margin=0.03):
+    """Tunes the batched path in two stages.
+
+    1. Every candidate whose closed-form FPR at `load` meets the budget runs a
+       scaled-down window (n_filters filters, n_batches batches).  Such a
+       window is dominated by per-level launch costs, which are nearly the
+       same for every configuration, so it only prunes.
+    2. The `shortlist` fastest, plus `baseline`, rerun on the full-size
+       window `confirm` = (filters, batches) with a different seed, best of
+       `repeats` timings each.  A candidate replaces the baseline only if it
+       is faster by more than `margin` (run-to-run spread of the window is a
+       few percent).
+
+    Returns (chosen kw, [(kw, stage-1 result)], [(kw, stage-2 result)])."""
+    if cands is None:
+        cands = candidates(bits=(16, 32), blocks=(256,))
+    cands = [kw for kw in cands if fpr_closed_form(kw["bits_per_tag"], kw["bucket_size"], load) <= fpr_budget]
+    results = [(kw, batched_trial(kw, n_filters, n_batches, seed, mix=mix, with_plain=False)) for kw in cands]
+    ranked = [kw for kw, _ in sorted(results, key=lambda x: -x[1]["batched_mkeys_s"])]
+    short = ranked[:shortlist]
+    if baseline is not None and baseline not in short:
+        short.append(baseline)
+    confirmed = [(kw, batched_trial(kw, confirm[0], confirm[1], seed + 1, mix=mix, with_plain=False,
+                                    repeats=repeats)) for kw in short]
+    if not confirmed:
+        return baseline, results, confirmed
+    best, best_r = max(confirmed, key=lambda x: x[1]["batched_mkeys_s"])
+    if baseline is not None:
+        base_r = next(r for kw, r in confirmed if kw == baseline)
+        if best_r["batched_mkeys_s"] < (1.0 + margin) * base_r["batched_mkeys_s"]:
+            best = baseline
+    return best, results, confirmed
+
+
+def sweep(slots, load, key_bytes=8, policy="xor", **axes):
+    """Measures every candidate; yields one result dict per point."""
+    for kw in candidates(key_bytes=key_bytes, policy=policy, **axes):
+        yield measure(kw, slots, load)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--slots", type=int, default=1 << 24)
+    ap.add_argument("--load", type=float, default=0.98)
+    ap.add_argument("--key-bytes", type=int, default=8)
+    ap.add_argument("--policy", default="xor")
+    ap.add_argument("--blocks", default="64,128,256,512")
+    ap.add_argument("--fpr-budget", type=float, default=None, help="also print the chosen point")
+    args = ap.parse_args(argv)
+    blocks = tuple(int(b) for b in args.blocks.split(","))
+    results = []
+    for r in sweep(args.slots, args.load, key_bytes=args.key_bytes, policy=args.policy, blocks=blocks):
+        results.append(r)
+        print(json.dumps(r), flush=True)
+    if args.fpr_budget is not None:
+        print(json.dumps({"chosen": choose(results, args.fpr_budget)}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -79,6 +79,14 @@ class cf_config(C.Structure):
 
 
 def _load():
+    # One HIP runtime per process: torch ships its own libamdhip64, and the
+    # library resolves against whichever copy is already loaded.  Loading the
+    # library first would bring in a second runtime, after which torch finds
+    # no device; so torch (when installed) is imported first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run python cuckoo-filter_amd/build.py "
                           "(or __graft_entry__.build()) first")
