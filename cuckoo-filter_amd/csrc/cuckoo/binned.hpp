@@ -67,6 +67,32 @@ struct EntrySource {
     }
 };
 
+/// First pass of the two-level partition: the entry's bucket bits below the
+/// first digit, beside the fingerprint, in 32 bits.
+struct PackBelowDigit {
+    using Out = uint32_t;
+    uint64_t bucketMask;
+    int tagBits;
+    __device__ __forceinline__ uint32_t operator()(uint64_t e) const {
+        const uint64_t fp = e & ((uint64_t(1) << tagBits) - 1);
+        return static_cast<uint32_t>((((e >> 32) & bucketMask) << tagBits) | fp);
+    }
+};
+
+/// Second pass: the bucket's position inside its slice, beside the fingerprint.
+struct KeepLowBits {
+    using Out = uint32_t;
+    uint32_t mask;
+    __device__ __forceinline__ uint32_t operator()(uint32_t v) const { return v & mask; }
+};
+
+/// Zeroes the three list counters of the insert (overflow, eviction starts,
+/// DFS list).
+template <typename C = unsigned long long>
+__global__ __launch_bounds__(64) void zeroCountersKernel(C* counters) {
+    if (threadIdx.x < 3) counters[threadIdx.x] = 0;
+}
+
 template <typename Src>
 __global__ __launch_bounds__(1024) void packEntriesKernel(Src src, size_t n, uint64_t* __restrict__ out) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
@@ -105,9 +131,12 @@ inline constexpr uint32_t kBinDepth = 4;
 inline constexpr size_t binOverflowOffset(size_t sliceBytes) { return (sliceBytes + 15) / 16 * 16; }
 inline size_t binLdsBytes(size_t sliceBytes) { return binOverflowOffset(sliceBytes) + kOverflowLds * sizeof(uint64_t); }
 
-/// Step 2: primary-bucket attempts of one slice's entries inside LDS.
-template <typename Config>
-__global__ __launch_bounds__(kBinBlock) void binInsertKernel(const uint64_t* __restrict__ entries,
+/// Step 2: primary-bucket attempts of one slice's entries inside LDS.  An
+/// entry E is either (bucket << 32 | fingerprint) (uint64_t) or, from the
+/// two-level partition, (bucket inside the slice << tag bits | fingerprint)
+/// (uint32_t).
+template <typename Config, typename E = uint64_t>
+__global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict__ entries,
                                                              const uint32_t* __restrict__ start, int log2Slice,
                                                              DeviceView<Config> v, uint64_t* __restrict__ overflow,
                                                              unsigned long long* overflowCount) {
@@ -150,7 +179,7 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const uint64_t* __r
     // kBinDepth entries per thread per step, loaded together (each load is
     // coalesced across the wave), then inserted one after another
     for (uint32_t e0 = lo + threadIdx.x; e0 < hi; e0 += kBinBlock * kBinDepth) {
-        uint64_t ent[kBinDepth];
+        E ent[kBinDepth];
 #pragma unroll
         for (uint32_t k = 0; k < kBinDepth; ++k) {
             const uint32_t e = e0 + k * kBinBlock;
@@ -159,11 +188,16 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const uint64_t* __r
 #pragma unroll
         for (uint32_t k = 0; k < kBinDepth; ++k) {
             const bool valid = e0 + k * kBinBlock < hi;
-            const uint64_t entry = ent[k];
+            const Tag fp = static_cast<Tag>(ent[k]);
+            size_t local;
+            if constexpr (sizeof(E) == 4) {
+                local = static_cast<size_t>(ent[k] >> (8 * sizeof(Tag)));
+            } else {
+                local = static_cast<size_t>(ent[k] >> 32) - b0;
+            }
+            const uint64_t entry = ((b0 + local) << 32) | static_cast<uint64_t>(fp);
             bool placed = true;
             if (valid) {
-                const size_t local = static_cast<size_t>(entry >> 32) - b0;
-                const Tag fp = static_cast<Tag>(entry);
                 Word* bw = slice + local * WC;
                 placed = Ops::tryInsertSnap(bw, detail::loadBucket<Word, WC>(bw), fp);
                 ok += placed;
@@ -348,6 +382,13 @@ inline InsertPath insertPathFromEnv() {
 
 inline constexpr size_t kBinnedMinBucketsPerKey = 4;
 
+/// CUCKOO_PARTITION=lsd keeps the one-level-per-pass LSD partition even where
+/// the two-level one applies (for comparisons).
+inline bool forceOnePassFromEnv() {
+    const char* e = std::getenv("CUCKOO_PARTITION");
+    return e != nullptr && std::strcmp(e, "lsd") == 0;
+}
+
 }  // namespace detail
 
 /// Host orchestration of the binned insert for one configuration.
@@ -376,8 +417,18 @@ struct BinnedInsert {
     static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
     static size_t workspaceBytes(size_t n, size_t numBuckets) {
-        return 2 * align256(n * sizeof(uint64_t)) + align256(radix::scratchBytes(n)) +
+        return 2 * align256(n * sizeof(uint64_t)) + align256(radix::twoLevelScratchBytes(n)) +
                align256((numSlices(numBuckets) + 1) * sizeof(uint32_t)) + align256(3 * sizeof(unsigned long long));
+    }
+
+    /// Whether the slice index splits into two digits whose second-level
+    /// entries fit in 32 bits: then the partition is two-level (MSD), and its
+    /// values are (bucket bits below the first digit, fingerprint) after the
+    /// first pass and (bucket inside the slice, fingerprint) after the second.
+    static constexpr int kTagBits = 8 * sizeof(typename Config::TagType);
+    static bool twoLevel(int l2s, int sliceBits) {
+        return sliceBits > radix::kDigitBits && sliceBits <= 2 * radix::kDigitBits &&
+               l2s + (sliceBits - radix::kDigitBits) + kTagBits <= 32 && !detail::forceOnePassFromEnv();
     }
 
     template <bool SortedSemantics>
@@ -396,12 +447,33 @@ struct BinnedInsert {
             uint64_t* b = reinterpret_cast<uint64_t*>(p);
             p += align256(m * sizeof(uint64_t));
             void* scratch = p;
-            p += align256(radix::scratchBytes(m));
+            p += align256(radix::twoLevelScratchBytes(m));
             uint32_t* start = reinterpret_cast<uint32_t*>(p);
             p += align256((slices + 1) * sizeof(uint32_t));
             unsigned long long* counters = reinterpret_cast<unsigned long long*>(p);  // overflow, evict, dfs
 
             EntrySource<Config, SortedSemantics> src{keys + off, v.numBuckets};
+            const size_t lds = binLdsBytes((size_t(1) << l2s) * kBucketBytes);
+            if (twoLevel(l2s, sliceBits)) {
+                // a holds the first pass's 32-bit values, b the second's; after
+                // the primary attempts the overflow list goes to a, eviction
+                // starts to b, the DFS list to a
+                const int d2 = sliceBits - radix::kDigitBits;
+                const PackBelowDigit pack1{(uint64_t(1) << (l2s + d2)) - 1, kTagBits};
+                const KeepLowBits pack2{(uint32_t(1) << (l2s + kTagBits)) - 1};
+                uint32_t* a32 = reinterpret_cast<uint32_t*>(a);
+                uint32_t* b32 = reinterpret_cast<uint32_t*>(b);
+                zeroCountersKernel<unsigned long long><<<1, 64, 0, s>>>(counters);
+                e = radix::partitionTwoLevel<uint64_t>(src, m, 32 + l2s + d2, pack1, l2s + kTagBits,
+                                                       uint32_t(1) << d2, pack2, a32, b32, scratch, start,
+                                                       static_cast<uint32_t>(slices), s);
+                if (e != hipSuccess) return e;
+                binInsertKernel<Config, uint32_t><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(
+                    b32, start, l2s, v, a, counters);
+                e = runStages(v, m, block, s, counters, a, b);
+                if (e != hipSuccess) return e;
+                continue;
+            }
             uint64_t* sorted = a;
             if (sliceBits > 0) {
                 e = radix::sortBits<uint64_t>(src, m, 32 + l2s, 32 + l2s + sliceBits, a, b, scratch, s, &sorted);
@@ -413,24 +485,31 @@ struct BinnedInsert {
             uint64_t* spare = sorted == a ? b : a;
             sliceStartKernel<uint64_t><<<static_cast<uint32_t>((slices + 1 + 255) / 256), 256, 0, s>>>(
                 sorted, static_cast<uint32_t>(m), 32 + l2s, static_cast<uint32_t>(slices), start, counters);
-            const size_t lds = binLdsBytes((size_t(1) << l2s) * kBucketBytes);
             // overflow list -> spare; eviction starts -> sorted (free by then); DFS list -> spare
             binInsertKernel<Config><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(sorted, start, l2s, v,
                                                                                        spare, counters);
-            const uint32_t stageGrid = detail::gridFor(secondaryKernel<Config>, (m + kStageChunk - 1) / kStageChunk * kStageBlock,
-                                                       kStageBlock);
-            secondaryKernel<Config><<<stageGrid, kStageBlock, 0, s>>>(spare, counters, sorted, counters + 1, v);
-            if (v.evictionPolicy == static_cast<uint32_t>(EvictionPolicy::BFS)) {
-                bfsStageKernel<Config><<<stageGrid, kStageBlock, 0, s>>>(sorted, counters + 1, spare, counters + 2, v);
-                auto k = dfsStageKernel<Config>;
-                k<<<detail::gridFor(k, m, block), block, 0, s>>>(spare, counters + 2, v);
-            } else {
-                auto k = dfsStageKernel<Config>;
-                k<<<detail::gridFor(k, m, block), block, 0, s>>>(sorted, counters + 1, v);
-            }
-            if ((e = hipGetLastError()) != hipSuccess) return e;
+            e = runStages(v, m, block, s, counters, spare, sorted);
+            if (e != hipSuccess) return e;
         }
         return hipSuccess;
+    }
+
+    /// Step 3 over the overflow list in `overflow`; `other` receives the
+    /// eviction starts, then `overflow` the DFS list.
+    static hipError_t runStages(const View& v, size_t m, uint32_t block, hipStream_t s, unsigned long long* counters,
+                                uint64_t* overflow, uint64_t* other) {
+        const uint32_t stageGrid = detail::gridFor(secondaryKernel<Config>,
+                                                   (m + kStageChunk - 1) / kStageChunk * kStageBlock, kStageBlock);
+        secondaryKernel<Config><<<stageGrid, kStageBlock, 0, s>>>(overflow, counters, other, counters + 1, v);
+        if (v.evictionPolicy == static_cast<uint32_t>(EvictionPolicy::BFS)) {
+            bfsStageKernel<Config><<<stageGrid, kStageBlock, 0, s>>>(other, counters + 1, overflow, counters + 2, v);
+            auto k = dfsStageKernel<Config>;
+            k<<<detail::gridFor(k, m, block), block, 0, s>>>(overflow, counters + 2, v);
+        } else {
+            auto k = dfsStageKernel<Config>;
+            k<<<detail::gridFor(k, m, block), block, 0, s>>>(other, counters + 1, v);
+        }
+        return hipGetLastError();
     }
 };
 

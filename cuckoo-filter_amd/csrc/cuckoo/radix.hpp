@@ -21,6 +21,12 @@
 // scatter runs as a resident grid that loads its next tile while it hashes,
 // ranks and writes the current one.  Later passes read arrays and scatter one
 // tile per workgroup.
+//
+// partitionTwoLevel is the MSD variant for two digits: after the first pass
+// each partition is split by the second digit on its own, so the bits of
+// the first digit can be dropped from the values (a scatter may write its
+// values through an output transform), and the group starts come out of
+// the second pass's scan.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -217,6 +223,14 @@ __device__ __forceinline__ uint64_t digitPeers(uint32_t d, bool valid) {
     return peers;
 }
 
+/// Output transform of a scatter: the value written for a staged value
+/// (the digit is always taken from the staged value).  Identity by default.
+template <typename U>
+struct Identity {
+    using Out = U;
+    __device__ __forceinline__ Out operator()(U v) const { return v; }
+};
+
 /// Dynamic LDS bytes the scatter kernel needs for value type U.
 template <typename U>
 constexpr size_t scatterLdsBytes() {
@@ -228,11 +242,10 @@ constexpr size_t scatterLdsBytes() {
 /// and issues the loads of its next tile before the current tile is ranked,
 /// staged and written, so the loads overlap the arithmetic and LDS work.  It
 /// is held to 80 VGPRs so that three workgroups (the LDS limit) fit per CU.
-template <typename U, typename Src>
-__global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, U* __restrict__ out, size_t n,
-                                                                         int shift,
-                                                                         const uint32_t* __restrict__ offsets,
-                                                                         uint32_t numTiles) {
+template <typename U, typename Src, typename P = Identity<U>>
+__global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, typename P::Out* __restrict__ out, size_t n,
+                                                           int shift, const uint32_t* __restrict__ offsets,
+                                                           uint32_t numTiles, P pack = P{}) {
     using Raw = typename Src::Raw;
     extern __shared__ __align__(16) unsigned char ldsRaw[];
     U* stage = reinterpret_cast<U*>(ldsRaw);
@@ -331,9 +344,77 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, U* __restric
         for (uint32_t j = threadIdx.x; j < tileCount; j += kBlock) {
             const U v = stage[j];
             const uint32_t d = static_cast<uint32_t>(v >> shift) & (kRadix - 1);
-            out[static_cast<uint32_t>(runShift[d] + j)] = v;  // exact: the sum is below n < 2^32
+            out[static_cast<uint32_t>(runShift[d] + j)] = pack(v);  // exact: the sum is below n < 2^32
         }
         __syncthreads();  // stage, cursor and the bases are reused by the next tile
+    }
+}
+
+/// Steps 1-4 of a one-tile scatter, shared by the array and segmented passes.
+/// `vals` holds this lane's values of the tile (row r of wave w is tile
+/// positions (w * kRowsPerWave + r) * 64 + lane), of which the first
+/// `tileCount` are valid; `rowOffsets` is the tile's row of global digit
+/// offsets.  `cursor` must be zero on entry (and a barrier passed since).
+template <typename U, typename P>
+__device__ __forceinline__ void scatterOneTile(const U (&vals)[kRowsPerWave], uint32_t tileCount, int shift,
+                                               const uint32_t* __restrict__ rowOffsets, P pack,
+                                               typename P::Out* __restrict__ out, U* stage,
+                                               uint32_t (*cursor)[kRadix], uint32_t* runShift) {
+    const int lane = __lane_id();
+    const int wave = threadIdx.x / 64;
+    const uint32_t waveBase = static_cast<uint32_t>(wave) * kRowsPerWave * 64;
+    const uint64_t ltMask = (1ull << lane) - 1;
+    static_assert(kRowsPerWave * kDigitBits <= 64, "digits of a lane must pack into 64 bits");
+    // 1. count digits per wave (the ballot ranking loop stays rolled, with the
+    //    lane's digits packed into one register)
+    uint64_t digits = 0;
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        digits |= static_cast<uint64_t>(static_cast<uint32_t>(vals[r] >> shift) & (kRadix - 1)) << (r * kDigitBits);
+    }
+#pragma unroll 1
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const bool valid = waveBase + static_cast<uint32_t>(r) * 64 + lane < tileCount;
+        const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
+        const uint64_t peers = digitPeers(d, valid);
+        if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
+    }
+    __syncthreads();
+    // 2. digit start inside the tile, each wave's offset inside the digit, and the global base
+    uint32_t tileTotal = 0;
+    if (threadIdx.x < kRadix) {
+        const int d = threadIdx.x;
+        for (int w = 0; w < kWaves; ++w) tileTotal += cursor[w][d];
+    }
+    uint32_t total;
+    const uint32_t start = blockExclusiveScan<kBlock>(threadIdx.x < kRadix ? tileTotal : 0u, total);
+    if (threadIdx.x < kRadix) {
+        const int d = threadIdx.x;
+        runShift[d] = rowOffsets[d] - start;
+        uint32_t run = start;
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = cursor[w][d];
+            cursor[w][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    // 3. stage values digit-ordered (stable) in LDS
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const bool valid = waveBase + static_cast<uint32_t>(r) * 64 + lane < tileCount;
+        const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
+        const uint64_t peers = digitPeers(d, valid);
+        const uint32_t pos = cursor[wave][d] + static_cast<uint32_t>(__popcll(peers & ltMask));
+        if (valid) stage[pos] = vals[r];
+        if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
+    }
+    __syncthreads();
+    // 4. contiguous runs per digit to their global place
+    for (uint32_t j = threadIdx.x; j < tileCount; j += kBlock) {
+        const U v = stage[j];
+        const uint32_t d = static_cast<uint32_t>(v >> shift) & (kRadix - 1);
+        out[static_cast<uint32_t>(runShift[d] + j)] = pack(v);  // exact: the sum is below n < 2^32
     }
 }
 
@@ -354,82 +435,166 @@ __global__ __launch_bounds__(kBlock) void scatterTileKernel(Src src, U* __restri
 
     const size_t tileBase = static_cast<size_t>(blockIdx.x) * kTile;
     const size_t waveBase = tileBase + static_cast<size_t>(wave) * kRowsPerWave * 64;
-    const uint64_t ltMask = (1ull << lane) - 1;
-    const size_t tileCount = n - tileBase < static_cast<size_t>(kTile) ? n - tileBase : static_cast<size_t>(kTile);
-
-    // 1. count digits per wave.  All rows of a lane are loaded first (many
-    //    loads in flight per wave); the ballot ranking loop stays rolled, with
-    //    the lane's digits packed into one register.
-    static_assert(kRowsPerWave * kDigitBits <= 64, "digits of a lane must pack into 64 bits");
+    const uint32_t tileCount =
+        static_cast<uint32_t>(n - tileBase < static_cast<size_t>(kTile) ? n - tileBase : static_cast<size_t>(kTile));
+    // all rows of a lane are loaded first (many loads in flight per wave)
     U vals[kRowsPerWave];
 #pragma unroll
     for (int r = 0; r < kRowsPerWave; ++r) {
         const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
         vals[r] = idx < n ? static_cast<U>(src(idx)) : U(0);
     }
-    uint64_t digits = 0;
-#pragma unroll
-    for (int r = 0; r < kRowsPerWave; ++r) {
-        digits |= static_cast<uint64_t>(static_cast<uint32_t>(vals[r] >> shift) & (kRadix - 1)) << (r * kDigitBits);
-    }
-#pragma unroll 1
-    for (int r = 0; r < kRowsPerWave; ++r) {
-        const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
-        const bool valid = idx < n;
-        const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
-        const uint64_t peers = digitPeers(d, valid);
-        if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
-    }
-    __syncthreads();
-    // 2. digit start inside the tile, each wave's offset inside the digit, and the global base
-    uint32_t tileTotal = 0;
-    if (threadIdx.x < kRadix) {
-        const int d = threadIdx.x;
-        for (int w = 0; w < kWaves; ++w) tileTotal += cursor[w][d];
-    }
+    scatterOneTile<U, Identity<U>>(vals, tileCount, shift, offsets + static_cast<size_t>(blockIdx.x) * kRadix,
+                                   Identity<U>{}, out, stage, cursor, runShift);
+}
+
+// Segmented pass: after a first pass has grouped the values into up to
+// kRadix partitions (the most significant digit), each partition is split
+// by the next digit on its own.  Tiles never straddle a partition, so the
+// next digit's offsets are per partition and the partition's own bits need
+// not be kept in the values.  Tile t belongs to the partition p with
+// tileBase[p] <= t < tileBase[p + 1] and covers partStart[p] + (t -
+// tileBase[p]) * kTile onwards, up to the partition's end.
+
+/// partStart[p] (p <= kRadix, partStart[kRadix] = n) and tileBase[p] from the
+/// first pass's offset matrix (row 0 holds every digit's global start).
+template <typename C = uint32_t>
+__global__ __launch_bounds__(kRadix) void segmentKernel(const C* __restrict__ firstOffsets, uint32_t n,
+                                                        C* __restrict__ partStart, C* __restrict__ tileBase) {
+    const uint32_t p = threadIdx.x;
+    const uint32_t s = firstOffsets[p];
+    const uint32_t e = p + 1 < kRadix ? firstOffsets[p + 1] : n;
+    const uint32_t tiles = (e - s + kTile - 1) / kTile;
     uint32_t total;
-    const uint32_t start = blockExclusiveScan<kBlock>(threadIdx.x < kRadix ? tileTotal : 0u, total);
-    if (threadIdx.x < kRadix) {
-        const int d = threadIdx.x;
-        runShift[d] = offsets[static_cast<size_t>(blockIdx.x) * kRadix + d] - start;
-        uint32_t run = start;
-        for (int w = 0; w < kWaves; ++w) {
-            const uint32_t c = cursor[w][d];
-            cursor[w][d] = run;
-            run += c;
+    const uint32_t before = blockExclusiveScan<kRadix>(tiles, total);
+    partStart[p] = s;
+    tileBase[p] = before;
+    if (p == 0) {
+        partStart[kRadix] = n;
+        tileBase[kRadix] = total;
+    }
+}
+
+/// Partition p and tile range [lo, hi) of segmented tile t; false if t is
+/// past the last tile.
+__device__ __forceinline__ bool segmentTile(const uint32_t* __restrict__ partStart,
+                                            const uint32_t* __restrict__ tileBase, uint32_t t, uint32_t& p,
+                                            uint32_t& lo, uint32_t& hi) {
+    if (t >= tileBase[kRadix]) return false;
+    uint32_t a = 0, b = kRadix;  // the last p with tileBase[p] <= t
+    while (b - a > 1) {
+        const uint32_t mid = (a + b) / 2;
+        if (tileBase[mid] <= t) {
+            a = mid;
+        } else {
+            b = mid;
         }
     }
+    p = a;
+    lo = partStart[p] + (t - tileBase[p]) * kTile;
+    const uint32_t end = partStart[p + 1];
+    hi = end - lo < static_cast<uint32_t>(kTile) ? end : lo + kTile;
+    return true;
+}
+
+/// Digit histogram of each segmented tile, one row of the tile-major matrix
+/// per tile (only the first `radix` columns are written).
+template <typename U>
+__global__ __launch_bounds__(kBlock) void countSegKernel(const U* __restrict__ in, const uint32_t* __restrict__ partStart,
+                                                         const uint32_t* __restrict__ tileBase, int shift, uint32_t radix,
+                                                         uint32_t* __restrict__ counts) {
+    __shared__ uint32_t hist[kRadix];
+    uint32_t p, lo, hi;
+    if (!segmentTile(partStart, tileBase, blockIdx.x, p, lo, hi)) return;
+    for (int i = threadIdx.x; i < kRadix; i += kBlock) hist[i] = 0;
     __syncthreads();
-    // 3. stage values digit-ordered (stable) in LDS
+    const uint32_t last = hi - 1;
+    U raw[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint32_t idx = lo + static_cast<uint32_t>(k) * kBlock + threadIdx.x;
+        raw[k] = in[idx < last ? idx : last];
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint32_t idx = lo + static_cast<uint32_t>(k) * kBlock + threadIdx.x;
+        if (idx < hi) atomicAdd(&hist[static_cast<uint32_t>(raw[k] >> shift) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < radix; i += kBlock) counts[static_cast<size_t>(blockIdx.x) * kRadix + i] = hist[i];
+}
+
+/// Per partition (one workgroup each): the digit-major exclusive scan of its
+/// tiles' rows, offset by the partition's start, written back in place; and
+/// the start of every (partition, digit) run as groupStart[p * radix + d]
+/// for the groups below numGroups (groupStart[numGroups] = n).
+template <typename C = uint32_t>
+__global__ __launch_bounds__(kRadix) void segScanKernel(C* __restrict__ counts, const C* __restrict__ partStart,
+                                                        const C* __restrict__ tileBase, uint32_t radix,
+                                                        C* __restrict__ groupStart, uint32_t numGroups) {
+    const uint32_t p = blockIdx.x;
+    const uint32_t d = threadIdx.x;
+    const uint32_t t0 = tileBase[p];
+    const uint32_t t1 = tileBase[p + 1];
+    uint32_t sum = 0;
+    if (d < radix) {
+#pragma unroll 8
+        for (uint32_t t = t0; t < t1; ++t) sum += counts[static_cast<size_t>(t) * kRadix + d];
+    }
+    uint32_t total;
+    uint32_t run = partStart[p] + blockExclusiveScan<kRadix>(sum, total);
+    if (d < radix) {
+        const uint32_t g = p * radix + d;
+        if (g < numGroups) groupStart[g] = run;
+#pragma unroll 8
+        for (uint32_t t = t0; t < t1; ++t) {
+            C* q = counts + static_cast<size_t>(t) * kRadix + d;
+            const uint32_t v = *q;
+            *q = run;
+            run += v;
+        }
+    }
+    if (p == 0 && d == 0) groupStart[numGroups] = partStart[kRadix];
+}
+
+/// The segmented pass's scatter: one segmented tile per workgroup.
+template <typename U, typename P>
+__global__ __launch_bounds__(kBlock) void scatterSegKernel(const U* __restrict__ in, typename P::Out* __restrict__ out,
+                                                           const uint32_t* __restrict__ partStart,
+                                                           const uint32_t* __restrict__ tileBase, int shift,
+                                                           const uint32_t* __restrict__ offsets, P pack) {
+    extern __shared__ __align__(16) unsigned char ldsRaw[];
+    U* stage = reinterpret_cast<U*>(ldsRaw);
+    __shared__ uint32_t cursor[kWaves][kRadix];
+    __shared__ uint32_t runShift[kRadix];
+    uint32_t p, lo, hi;
+    if (!segmentTile(partStart, tileBase, blockIdx.x, p, lo, hi)) return;
+    for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&cursor[0][0])[i] = 0;
+    __syncthreads();
+    const int lane = __lane_id();
+    const int wave = threadIdx.x / 64;
+    const uint32_t waveBase = lo + static_cast<uint32_t>(wave) * kRowsPerWave * 64;
+    const uint32_t last = hi - 1;
+    U vals[kRowsPerWave];
 #pragma unroll
     for (int r = 0; r < kRowsPerWave; ++r) {
-        const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
-        const bool valid = idx < n;
-        const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
-        const uint64_t peers = digitPeers(d, valid);
-        const uint32_t pos = cursor[wave][d] + static_cast<uint32_t>(__popcll(peers & ltMask));
-        if (valid) stage[pos] = vals[r];
-        if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
+        const uint32_t idx = waveBase + static_cast<uint32_t>(r) * 64 + lane;
+        vals[r] = in[idx < last ? idx : last];  // past-the-end lanes are masked by tileCount
     }
-    __syncthreads();
-    // 4. contiguous runs per digit to their global place
-    for (uint32_t j = threadIdx.x; j < tileCount; j += kBlock) {
-        const U v = stage[j];
-        const uint32_t d = static_cast<uint32_t>(v >> shift) & (kRadix - 1);
-        out[static_cast<uint32_t>(runShift[d] + j)] = v;  // exact: the sum is below n < 2^32
-    }
+    scatterOneTile<U, P>(vals, hi - lo, shift, offsets + static_cast<size_t>(blockIdx.x) * kRadix, pack, out, stage,
+                         cursor, runShift);
 }
 
 /// Grid of the pipelined scatter kernel: its resident workgroups on the
 /// current device (at most one per tile).
-template <typename U, typename Src>
+template <typename U, typename Src, typename P = Identity<U>>
 inline uint32_t scatterGrid(uint32_t numTiles) {
     int dev = 0;
     int cus = 0;
     int perCu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, scatterKernel<U, Src>, kBlock,
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, scatterKernel<U, Src, P>, kBlock,
                                                      scatterLdsBytes<U>()) !=
             hipSuccess ||
         cus <= 0 || perCu <= 0) {
@@ -450,6 +615,58 @@ inline size_t scratchBytes(size_t n) {
     return (m + cm + parts) * sizeof(uint32_t) + 256;
 }
 
+/// Digit-major exclusive scan of a tile-major count matrix, in place.
+inline void scanCounts(uint32_t* counts, uint32_t tiles, uint32_t* chunkSum, uint32_t* partial, hipStream_t stream) {
+    const uint32_t chunks = (tiles + kChunkTiles - 1) / kChunkTiles;
+    const size_t cm = static_cast<size_t>(chunks) * kRadix;
+    const uint32_t parts = static_cast<uint32_t>((cm + kScanChunk - 1) / kScanChunk);
+    chunkSumKernel<uint32_t><<<chunks, kRadix, 0, stream>>>(counts, tiles, chunkSum, chunks);
+    scanReduceKernel<uint32_t><<<parts, kScanBlock, 0, stream>>>(chunkSum, cm, partial);
+    scanPartialsKernel<uint32_t><<<1, kScanBlock, 0, stream>>>(partial, parts);
+    scanDownKernel<uint32_t><<<parts, kScanBlock, 0, stream>>>(chunkSum, cm, partial);
+    chunkDownKernel<uint32_t><<<chunks, kRadix, 0, stream>>>(counts, tiles, chunkSum, chunks);
+}
+
+/// Scratch of partitionTwoLevel for n values, in bytes.
+inline size_t twoLevelScratchBytes(size_t n) {
+    // the second pass's count rows (up to kRadix more tiles), then the partition starts and tile bases
+    return scratchBytes(n) + static_cast<size_t>(kRadix) * kRadix * sizeof(uint32_t) +
+           2 * (kRadix + 1) * sizeof(uint32_t);
+}
+
+/// Two-level (MSD) partition of n values into groups p * radix2 + d:
+///  1. the values computed by `first` are split by the kDigitBits-wide digit
+///     at `shift1` into partitions p and written to `a` through `pack1`;
+///  2. each partition is split by the digit at `shift2` of the packed value
+///     (radix2 <= kRadix values) and written to `b` through `pack2`.
+/// groupStart[g] receives the start of group g for g < numGroups, and
+/// groupStart[numGroups] = n.  n must be below 2^31.
+template <typename U, typename Src, typename P1, typename P2>
+inline hipError_t partitionTwoLevel(Src first, size_t n, int shift1, P1 pack1, int shift2, uint32_t radix2, P2 pack2,
+                                    typename P1::Out* a, typename P2::Out* b, void* scratch, uint32_t* groupStart,
+                                    uint32_t numGroups, hipStream_t stream) {
+    using V = typename P1::Out;
+    if (n == 0) return hipSuccess;
+    const uint32_t tiles = static_cast<uint32_t>((n + kTile - 1) / kTile);
+    const uint32_t tiles2 = tiles + kRadix;  // each partition adds at most one partial tile
+    const uint32_t chunks = (tiles + kChunkTiles - 1) / kChunkTiles;
+    uint32_t* counts = static_cast<uint32_t*>(scratch);                      // [tile][digit], both passes
+    uint32_t* chunkSum = counts + static_cast<size_t>(tiles2) * kRadix;      // [digit][chunk]
+    uint32_t* partial = chunkSum + static_cast<size_t>(chunks) * kRadix;
+    uint32_t* partStart = partial + (static_cast<size_t>(chunks) * kRadix + kScanChunk - 1) / kScanChunk;
+    uint32_t* tileBase = partStart + kRadix + 1;
+    countKernel<U, Src><<<tiles, kBlock, 0, stream>>>(first, n, shift1, counts, tiles);
+    scanCounts(counts, tiles, chunkSum, partial, stream);
+    scatterKernel<U, Src, P1><<<scatterGrid<U, Src, P1>(tiles), kBlock, scatterLdsBytes<U>(), stream>>>(
+        first, a, n, shift1, counts, tiles, pack1);
+    segmentKernel<uint32_t><<<1, kRadix, 0, stream>>>(counts, static_cast<uint32_t>(n), partStart, tileBase);
+    countSegKernel<V><<<tiles2, kBlock, 0, stream>>>(a, partStart, tileBase, shift2, radix2, counts);
+    segScanKernel<uint32_t><<<kRadix, kRadix, 0, stream>>>(counts, partStart, tileBase, radix2, groupStart, numGroups);
+    scatterSegKernel<V, P2><<<tiles2, kBlock, scatterLdsBytes<V>(), stream>>>(a, b, partStart, tileBase, shift2,
+                                                                               counts, pack2);
+    return hipGetLastError();
+}
+
 /// Sorts n values by bits [lo, hi), 8 bits per pass.  The first pass reads
 /// through `first` (which may compute the values); later passes ping-pong
 /// between a and b.  *result receives the buffer holding the sorted values.
@@ -462,11 +679,9 @@ inline hipError_t sortBits(Src first, size_t n, int lo, int hi, U* a, U* b, void
     const uint32_t tiles = static_cast<uint32_t>((n + kTile - 1) / kTile);
     const size_t m = static_cast<size_t>(tiles) * kRadix;
     const uint32_t chunks = (tiles + kChunkTiles - 1) / kChunkTiles;
-    const size_t cm = static_cast<size_t>(chunks) * kRadix;
-    const uint32_t parts = static_cast<uint32_t>((cm + kScanChunk - 1) / kScanChunk);
-    uint32_t* counts = static_cast<uint32_t*>(scratch);  // [tile][digit]
-    uint32_t* chunkSum = counts + m;                      // [digit][chunk]
-    uint32_t* partial = chunkSum + cm;
+    uint32_t* counts = static_cast<uint32_t*>(scratch);                  // [tile][digit]
+    uint32_t* chunkSum = counts + m;                                      // [digit][chunk]
+    uint32_t* partial = chunkSum + static_cast<size_t>(chunks) * kRadix;  // chunk-sum partials
     U* dst = a;
     U* other = b;
     bool firstPass = true;
@@ -477,11 +692,7 @@ inline hipError_t sortBits(Src first, size_t n, int lo, int hi, U* a, U* b, void
             countKernel<U, ArraySource<U>><<<tiles, kBlock, 0, stream>>>(ArraySource<U>{other}, n, shift, counts,
                                                                           tiles);
         }
-        chunkSumKernel<uint32_t><<<chunks, kRadix, 0, stream>>>(counts, tiles, chunkSum, chunks);
-        scanReduceKernel<uint32_t><<<parts, kScanBlock, 0, stream>>>(chunkSum, cm, partial);
-        scanPartialsKernel<uint32_t><<<1, kScanBlock, 0, stream>>>(partial, parts);
-        scanDownKernel<uint32_t><<<parts, kScanBlock, 0, stream>>>(chunkSum, cm, partial);
-        chunkDownKernel<uint32_t><<<chunks, kRadix, 0, stream>>>(counts, tiles, chunkSum, chunks);
+        scanCounts(counts, tiles, chunkSum, partial, stream);
         if (firstPass) {
             scatterKernel<U, Src><<<scatterGrid<U, Src>(tiles), kBlock, scatterLdsBytes<U>(), stream>>>(
                 first, dst, n, shift, counts, tiles);

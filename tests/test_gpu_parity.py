@@ -274,6 +274,59 @@ def test_batch_insert_invariants_and_binomial_parity(cf, oracle_mod, kw, mode, m
     f.close()
 
 
+# Tables of 2^9 slices and more split the slice index into two digits.  Where
+# the bucket bits below the first digit fit beside the tag in 32 bits, the
+# binned insert partitions in two levels (MSD) with 32-bit entries; otherwise
+# (32-bit tags) it keeps the LSD passes with 64-bit entries.
+TWO_LEVEL_CFGS = [
+    (dict(eviction="bfs"), 1 << 23),                                      # 512 slices: digits 8 + 1
+    (dict(eviction="dfs"), 1 << 24),                                      # 8 + 2
+    (dict(eviction="bfs", bits_per_tag=8, word_bytes=4, bucket_size=16), 1 << 24),  # 16-byte buckets
+    (dict(eviction="bfs", bits_per_tag=8, bucket_size=4, word_bytes=4), 1 << 24),   # 4-byte buckets, 2^13-bucket slices
+    (dict(eviction="bfs", bits_per_tag=32, bucket_size=8), 1 << 23),      # too wide: LSD
+    (dict(eviction="bfs", policy="addsub"), 1 << 23),
+]
+
+
+@pytest.mark.parametrize("partition", ["default", "lsd"])
+@pytest.mark.parametrize("mode", ["binned", "sorted"])
+@pytest.mark.parametrize("kw,cap", TWO_LEVEL_CFGS, ids=lambda x: "-".join(f"{k}{v}" for k, v in x.items())
+                         if isinstance(x, dict) else f"cap{x}")
+def test_partition_levels_insert_invariants(cf, oracle_mod, kw, cap, mode, partition, monkeypatch):
+    import torch
+
+    monkeypatch.setenv("CUCKOO_INSERT_PATH", "binned")
+    if partition == "lsd":
+        monkeypatch.setenv("CUCKOO_PARTITION", "lsd")
+    c = cf.make_config(**kw)
+    f = cf.CuckooFilter(cap, config=c)
+    load = 0.95 if c.bucket_size >= 8 else 0.85
+    n = int(f.capacity() * load)
+    host = oracle_mod.gen_keys(77, n + (1 << 20), c.key_bytes)
+    keys = _dev_keys(cf, host)
+    pos, neg = keys[:n], keys[n:]
+    occ = f.insertManySorted(pos) if mode == "sorted" else f.insertMany(pos)
+    fails = n - occ
+    if fails:
+        # exhausted eviction chains: as many as the sequential oracle has, within a binomial band
+        of = oracle_mod.Filter(_oracle_cfg(oracle_mod, c), cap)
+        fails_cpu = int((~of.insert_many(host[:n])[1]).sum())
+        assert abs(fails - fails_cpu) <= 3 * np.sqrt(max(fails_cpu, 1)) + 3, (fails, fails_cpu)
+    assert_table_count(f, occ, n)
+    assert n - int(f.containsMany(pos).sum().item()) <= fails
+    # false positives: binomial band around the closed form at the reached load
+    p = 1 - (1 - 1 / (2 ** c.bits_per_tag - 1)) ** (2 * c.bucket_size * occ / f.capacity())
+    m = neg.numel()
+    hits = int(f.containsMany(neg).sum().item())
+    assert abs(hits - m * p) < 5 * np.sqrt(m * p * (1 - p)) + 3, (hits, m * p)
+    out = torch.zeros(n // 3, dtype=torch.bool, device=pos.device)
+    rem = f.deleteMany(pos[: n // 3], out=out)
+    deleted = int(out.sum().item())
+    assert deleted >= n // 3 - fails and rem == occ - deleted
+    assert n - n // 3 - int(f.containsMany(pos[n // 3:]).sum().item()) <= fails
+    f.close()
+
+
 @pytest.mark.parametrize("path", ["direct", "binned"])
 def test_eviction_histogram_accounts_for_every_insert(cf, oracle_mod, path, monkeypatch):
     monkeypatch.setenv("CUCKOO_INSERT_PATH", path)
