@@ -199,7 +199,7 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict
             bool placed = true;
             if (valid) {
                 Word* bw = slice + local * WC;
-                placed = Ops::tryInsertSnap(bw, detail::loadBucket<Word, WC>(bw), fp);
+                placed = Ops::tryInsertLds(bw, fp);
                 ok += placed;
             }
             const uint64_t fails = __ballot(!placed);

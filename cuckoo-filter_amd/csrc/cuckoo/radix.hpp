@@ -12,10 +12,12 @@
 //  * scan: the digit-major exclusive scan of that matrix, done over chunks of
 //    rows so that every access to it is a contiguous row;
 //  * scatter: a tile is split into 64-value rows, each wave owns a contiguous
-//    run of rows, a value's rank inside its row comes from eight ballots (one
-//    per digit bit); values are first placed digit-ordered in an LDS staging
+//    run of rows; values are first placed digit-ordered in an LDS staging
 //    tile, then written out so that each digit's run is a contiguous,
-//    coalesced store.  The scatter is stable, as LSD needs.
+//    coalesced store.  Passes after the first of an LSD partition must be
+//    stable: there a value's rank inside its row comes from eight ballots
+//    (one per digit bit).  A first pass need not be, and ranks each value
+//    with one LDS atomic on its digit's counter.
 // The first pass reads its values through a source functor, so the values
 // (e.g. entries computed from keys) need not be materialised beforehand; its
 // scatter runs as a resident grid that loads its next tile while it hashes,
@@ -237,6 +239,62 @@ constexpr size_t scatterLdsBytes() {
     return static_cast<size_t>(kTile) * sizeof(U);
 }
 
+/// Steps of an unstable one-tile scatter, for passes whose order inside a
+/// digit does not matter (the first pass of any partition, and the second
+/// pass of the two-level one).  A value's rank among the tile's values of
+/// its digit is the return of one LDS atomic on that digit's counter (kept
+/// in a register), so a value is ranked once, with no ballots.  `vals` holds
+/// this lane's values (row r of wave w is tile positions (w * kRowsPerWave +
+/// r) * 64 + lane), the first `tileCount` valid; `myBase` is, for threads
+/// below kRadix, the global offset of digit threadIdx.x for this tile.
+/// `hist` (kRadix counters) must be zero on entry, with a barrier passed.
+template <typename U, typename P>
+__device__ __forceinline__ void scatterTileUnstable(const U (&vals)[kRowsPerWave], uint32_t tileCount, int shift,
+                                                    uint32_t myBase, P pack, typename P::Out* __restrict__ out,
+                                                    U* stage, uint32_t* hist, uint32_t* runShift) {
+    const int lane = __lane_id();
+    const int wave = threadIdx.x / 64;
+    const uint32_t waveBase = static_cast<uint32_t>(wave) * kRowsPerWave * 64;
+    // 1. ranks (two 16-bit ranks per register: a tile has 4096 values)
+    uint32_t rank[kRowsPerWave / 2];
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const bool valid = waveBase + static_cast<uint32_t>(r) * 64 + lane < tileCount;
+        const uint32_t d = static_cast<uint32_t>(vals[r] >> shift) & (kRadix - 1);
+        const uint32_t k = valid ? atomicAdd(&hist[d], 1u) : 0u;
+        if (r % 2 == 0) {
+            rank[r / 2] = k;
+        } else {
+            rank[r / 2] |= k << 16;
+        }
+    }
+    __syncthreads();
+    // 2. digit starts inside the tile, and each digit's shift to its global place
+    const uint32_t c = threadIdx.x < kRadix ? hist[threadIdx.x] : 0u;
+    uint32_t total;
+    const uint32_t start = blockExclusiveScan<kBlock>(c, total);
+    if (threadIdx.x < kRadix) {
+        hist[threadIdx.x] = start;
+        runShift[threadIdx.x] = myBase - start;
+    }
+    __syncthreads();
+    // 3. stage values digit-ordered in LDS
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const bool valid = waveBase + static_cast<uint32_t>(r) * 64 + lane < tileCount;
+        const uint32_t d = static_cast<uint32_t>(vals[r] >> shift) & (kRadix - 1);
+        const uint32_t k = (rank[r / 2] >> (16 * (r % 2))) & 0xFFFFu;
+        if (valid) stage[hist[d] + k] = vals[r];
+    }
+    __syncthreads();
+    // 4. contiguous runs per digit to their global place
+    for (uint32_t j = threadIdx.x; j < tileCount; j += kBlock) {
+        const U v = stage[j];
+        const uint32_t d = static_cast<uint32_t>(v >> shift) & (kRadix - 1);
+        out[static_cast<uint32_t>(runShift[d] + j)] = pack(v);  // exact: the sum is below n < 2^32
+    }
+}
+
 /// Pipelined, for sources that compute their values (such as hashing keys):
 /// a resident grid in which workgroup b handles tiles b, b + gridDim.x, ...
 /// and issues the loads of its next tile before the current tile is ranked,
@@ -249,13 +307,11 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, typename P::
     using Raw = typename Src::Raw;
     extern __shared__ __align__(16) unsigned char ldsRaw[];
     U* stage = reinterpret_cast<U*>(ldsRaw);
-    __shared__ uint32_t cursor[kWaves][kRadix];
+    __shared__ uint32_t hist[kRadix];
     __shared__ uint32_t runShift[kRadix];  // global start of the digit's run minus its start in the tile
 
     const int lane = __lane_id();
     const int wave = threadIdx.x / 64;
-    const uint64_t ltMask = (1ull << lane) - 1;
-    static_assert(kRowsPerWave * kDigitBits <= 64, "digits of a lane must pack into 64 bits");
 
     uint32_t t = blockIdx.x;
     if (t >= numTiles) return;
@@ -272,9 +328,9 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, typename P::
     }
     for (; t < numTiles; t += gridDim.x) {
         const size_t tileBase = static_cast<size_t>(t) * kTile;
-        const size_t waveBase = tileBase + static_cast<size_t>(wave) * kRowsPerWave * 64;
-        const size_t tileCount = n - tileBase < static_cast<size_t>(kTile) ? n - tileBase : static_cast<size_t>(kTile);
-        for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&cursor[0][0])[i] = 0;
+        const uint32_t tileCount =
+            static_cast<uint32_t>(n - tileBase < static_cast<size_t>(kTile) ? n - tileBase : static_cast<size_t>(kTile));
+        if (threadIdx.x < kRadix) hist[threadIdx.x] = 0;
         // this tile's values; then the next tile's loads go out
         U vals[kRowsPerWave];
 #pragma unroll
@@ -293,60 +349,9 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, typename P::
                 raw[r] = src.load(idx < last ? idx : last);
             }
         }
-        uint64_t digits = 0;
-#pragma unroll
-        for (int r = 0; r < kRowsPerWave; ++r) {
-            digits |= static_cast<uint64_t>(static_cast<uint32_t>(vals[r] >> shift) & (kRadix - 1)) << (r * kDigitBits);
-        }
-        __syncthreads();  // cursor cleared
-        // 1. count digits per wave (ballot ranking, rolled)
-#pragma unroll 1
-        for (int r = 0; r < kRowsPerWave; ++r) {
-            const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
-            const bool valid = idx < n;
-            const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
-            const uint64_t peers = digitPeers(d, valid);
-            if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
-        }
-        __syncthreads();
-        // 2. digit start inside the tile, each wave's offset inside the digit, and the global base
-        uint32_t tileTotal = 0;
-        if (threadIdx.x < kRadix) {
-            const int d = threadIdx.x;
-            for (int w = 0; w < kWaves; ++w) tileTotal += cursor[w][d];
-        }
-        uint32_t total;
-        const uint32_t start = blockExclusiveScan<kBlock>(threadIdx.x < kRadix ? tileTotal : 0u, total);
-        if (threadIdx.x < kRadix) {
-            const int d = threadIdx.x;
-            runShift[d] = myBase - start;
-            uint32_t run = start;
-            for (int w = 0; w < kWaves; ++w) {
-                const uint32_t c = cursor[w][d];
-                cursor[w][d] = run;
-                run += c;
-            }
-        }
-        __syncthreads();
-        // 3. stage values digit-ordered (stable) in LDS
-#pragma unroll
-        for (int r = 0; r < kRowsPerWave; ++r) {
-            const size_t idx = waveBase + static_cast<size_t>(r) * 64 + lane;
-            const bool valid = idx < n;
-            const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
-            const uint64_t peers = digitPeers(d, valid);
-            const uint32_t pos = cursor[wave][d] + static_cast<uint32_t>(__popcll(peers & ltMask));
-            if (valid) stage[pos] = vals[r];
-            if (valid && (peers & ltMask) == 0) cursor[wave][d] += static_cast<uint32_t>(__popcll(peers));
-        }
-        __syncthreads();
-        // 4. contiguous runs per digit to their global place
-        for (uint32_t j = threadIdx.x; j < tileCount; j += kBlock) {
-            const U v = stage[j];
-            const uint32_t d = static_cast<uint32_t>(v >> shift) & (kRadix - 1);
-            out[static_cast<uint32_t>(runShift[d] + j)] = pack(v);  // exact: the sum is below n < 2^32
-        }
-        __syncthreads();  // stage, cursor and the bases are reused by the next tile
+        __syncthreads();  // hist cleared
+        scatterTileUnstable<U, P>(vals, tileCount, shift, myBase, pack, out, stage, hist, runShift);
+        __syncthreads();  // stage, hist and the bases are reused by the next tile
     }
 }
 
@@ -565,12 +570,11 @@ __global__ __launch_bounds__(kBlock) void scatterSegKernel(const U* __restrict__
                                                            const uint32_t* __restrict__ offsets, P pack) {
     extern __shared__ __align__(16) unsigned char ldsRaw[];
     U* stage = reinterpret_cast<U*>(ldsRaw);
-    __shared__ uint32_t cursor[kWaves][kRadix];
+    __shared__ uint32_t hist[kRadix];
     __shared__ uint32_t runShift[kRadix];
     uint32_t p, lo, hi;
     if (!segmentTile(partStart, tileBase, blockIdx.x, p, lo, hi)) return;
-    for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&cursor[0][0])[i] = 0;
-    __syncthreads();
+    if (threadIdx.x < kRadix) hist[threadIdx.x] = 0;
     const int lane = __lane_id();
     const int wave = threadIdx.x / 64;
     const uint32_t waveBase = lo + static_cast<uint32_t>(wave) * kRowsPerWave * 64;
@@ -581,8 +585,10 @@ __global__ __launch_bounds__(kBlock) void scatterSegKernel(const U* __restrict__
         const uint32_t idx = waveBase + static_cast<uint32_t>(r) * 64 + lane;
         vals[r] = in[idx < last ? idx : last];  // past-the-end lanes are masked by tileCount
     }
-    scatterOneTile<U, P>(vals, hi - lo, shift, offsets + static_cast<size_t>(blockIdx.x) * kRadix, pack, out, stage,
-                         cursor, runShift);
+    const uint32_t myBase =
+        offsets[static_cast<size_t>(blockIdx.x) * kRadix + (threadIdx.x < kRadix ? threadIdx.x : kRadix - 1)];
+    __syncthreads();  // hist cleared
+    scatterTileUnstable<U, P>(vals, hi - lo, shift, myBase, pack, out, stage, hist, runShift);
 }
 
 /// Grid of the pipelined scatter kernel: its resident workgroups on the
