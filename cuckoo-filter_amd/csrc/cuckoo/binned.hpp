@@ -124,17 +124,43 @@ __global__ __launch_bounds__(256) void sliceStartKernel(const T* __restrict__ en
 /// space with a single atomic (more than this spill with one atomic per wave).
 inline constexpr uint32_t kOverflowLds = 2048;
 
-/// Entries each thread of binInsertKernel loads at once.
+/// Entries each thread of binInsertKernel loads at once, and the entries a
+/// workgroup takes per round.
 inline constexpr uint32_t kBinDepth = 4;
+inline constexpr uint32_t kBinRound = kBinBlock * kBinDepth;
 
-/// Dynamic LDS of binInsertKernel: the table slice, then the overflow buffer.
+/// Dynamic LDS of binInsertKernel: the table slice, the overflow buffer, then
+/// the round's retry list (kBinRound entries of retryBytes each).
 inline constexpr size_t binOverflowOffset(size_t sliceBytes) { return (sliceBytes + 15) / 16 * 16; }
-inline size_t binLdsBytes(size_t sliceBytes) { return binOverflowOffset(sliceBytes) + kOverflowLds * sizeof(uint64_t); }
+inline constexpr size_t binRetryOffset(size_t sliceBytes) {
+    return binOverflowOffset(sliceBytes) + kOverflowLds * sizeof(uint64_t);
+}
+inline size_t binLdsBytes(size_t sliceBytes, size_t retryBytes) {
+    return binRetryOffset(sliceBytes) + kBinRound * retryBytes;
+}
+
+/// Retry-list entry of binInsertKernel: (bucket inside the slice << tag bits
+/// | fingerprint), in 32 bits where that fits.
+template <typename Config>
+using BinRetryOf = std::conditional_t<(sizeof(typename Config::TagType) <= 2), uint32_t, uint64_t>;
 
 /// Step 2: primary-bucket attempts of one slice's entries inside LDS.  An
 /// entry E is either (bucket << 32 | fingerprint) (uint64_t) or, from the
 /// two-level partition, (bucket inside the slice << tag bits | fingerprint)
 /// (uint32_t).
+///
+/// Each round of kBinRound entries runs in two phases, so that the lanes of
+/// a wave do the same work:
+///  1. every entry tries its tag's start word (one LDS word read, one CAS),
+///     which is where the reference's slot scan starts; entries whose start
+///     word is full, or whose CAS lost, go to an LDS retry list;
+///  2. the retry list, compacted into full waves, runs the rest of the
+///     reference's scan on the whole bucket from the start word on; entries
+///     that find the bucket full go to the overflow list.
+/// Deferring a key between its start word and the rest of its bucket is an
+/// interleaving the concurrent reference allows; a full word stays full while
+/// only inserts run, so the later scan sees what the reference's thread would
+/// see after a delay.
 template <typename Config, typename E = uint64_t>
 __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict__ entries,
                                                              const uint32_t* __restrict__ start, int log2Slice,
@@ -143,10 +169,13 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict
     using Ops = FilterOps<Config>;
     using Word = typename Config::WordType;
     using Tag = typename Config::TagType;
+    using R = BinRetryOf<Config>;
     constexpr size_t WC = Layout<Config>::wordCount;
     constexpr size_t kBucketBytes = Layout<Config>::bucketBytes;
+    constexpr int kTagBits = 8 * sizeof(Tag);
     extern __shared__ __align__(16) unsigned char ldsRaw[];
     __shared__ uint32_t ovCount;
+    __shared__ uint32_t retryCount[2];  // by round parity
     __shared__ unsigned long long ovBase;
 
     const uint32_t s = blockIdx.x;
@@ -157,13 +186,18 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict
     const size_t sliceBuckets = static_cast<size_t>(1) << log2Slice;
     Word* slice = reinterpret_cast<Word*>(ldsRaw);
     uint64_t* ovBuf = reinterpret_cast<uint64_t*>(ldsRaw + binOverflowOffset(sliceBuckets * kBucketBytes));
+    R* retry = reinterpret_cast<R*>(ldsRaw + binRetryOffset(sliceBuckets * kBucketBytes));
     const size_t b0 = static_cast<size_t>(s) << log2Slice;
     const size_t nbs = v.numBuckets - b0 < sliceBuckets ? v.numBuckets - b0 : sliceBuckets;
     const size_t words = nbs * WC;
     const Word* g = v.words + b0 * WC;
     Word* gw = v.words + b0 * WC;
 
-    if (threadIdx.x == 0) ovCount = 0;
+    if (threadIdx.x == 0) {
+        ovCount = 0;
+        retryCount[0] = 0;
+        retryCount[1] = 0;
+    }
     using V4 = unsigned int __attribute__((ext_vector_type(4)));
     const size_t bytes = words * sizeof(Word);
     const size_t vecs = bytes / 16;
@@ -176,41 +210,69 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict
     uint32_t ok = 0;
     const int lane = __lane_id();
     const uint64_t ltMask = (1ull << lane) - 1;
-    // kBinDepth entries per thread per step, loaded together (each load is
-    // coalesced across the wave), then inserted one after another
-    for (uint32_t e0 = lo + threadIdx.x; e0 < hi; e0 += kBinBlock * kBinDepth) {
+    const uint32_t last = hi - 1;
+    uint32_t parity = 0;
+    for (uint32_t r0 = lo; r0 < hi; r0 += kBinRound, parity ^= 1u) {
+        // 1. start words; kBinDepth entries per thread, loaded together (each
+        //    load coalesced across the wave; past-the-end lanes reread the last)
         E ent[kBinDepth];
 #pragma unroll
         for (uint32_t k = 0; k < kBinDepth; ++k) {
-            const uint32_t e = e0 + k * kBinBlock;
-            ent[k] = e < hi ? entries[e] : 0;
+            const uint32_t e = r0 + k * kBinBlock + threadIdx.x;
+            ent[k] = entries[e < last ? e : last];
         }
 #pragma unroll
         for (uint32_t k = 0; k < kBinDepth; ++k) {
-            const bool valid = e0 + k * kBinBlock < hi;
+            const bool valid = r0 + k * kBinBlock + threadIdx.x < hi;
             const Tag fp = static_cast<Tag>(ent[k]);
-            size_t local;
+            uint32_t local;
             if constexpr (sizeof(E) == 4) {
-                local = static_cast<size_t>(ent[k] >> (8 * sizeof(Tag)));
+                local = static_cast<uint32_t>(ent[k] >> kTagBits);
             } else {
-                local = static_cast<size_t>(ent[k] >> 32) - b0;
+                local = static_cast<uint32_t>(static_cast<size_t>(ent[k] >> 32) - b0);
             }
-            const uint64_t entry = ((b0 + local) << 32) | static_cast<uint64_t>(fp);
+            bool again = false;
+            if (valid) {
+                const bool placed = Ops::tryStartWordLds(slice + static_cast<size_t>(local) * WC, fp);
+                ok += placed;
+                again = !placed;
+            }
+            const uint64_t mask = __ballot(again);
+            if (mask != 0) {
+                // one LDS atomic per wave reserves this wave's run of the retry list
+                const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&retryCount[parity], static_cast<uint32_t>(__popcll(mask)));
+                base = __shfl(base, leader, 64);
+                if (again) retry[base + __popcll(mask & ltMask)] = (static_cast<R>(local) << kTagBits) | static_cast<R>(fp);
+            }
+        }
+        __syncthreads();
+        // 2. the rest of the bucket for the round's retries, in full waves
+        const uint32_t nr = retryCount[parity];
+        if (threadIdx.x == 0) retryCount[parity ^ 1u] = 0;  // the next round's list (last read a round ago)
+        for (uint32_t j0 = 0; j0 < nr; j0 += kBinBlock) {
+            const uint32_t j = j0 + threadIdx.x;
+            const bool valid = j < nr;
+            const R rv = valid ? retry[j] : R(0);
+            const uint32_t local = static_cast<uint32_t>(rv >> kTagBits);
+            const Tag fp = static_cast<Tag>(rv);
             bool placed = true;
             if (valid) {
-                Word* bw = slice + local * WC;
-                placed = Ops::tryInsertLds(bw, fp);
+                Word* bw = slice + static_cast<size_t>(local) * WC;
+                placed = Ops::tryInsertSnap(bw, detail::loadBucket<Word, WC>(bw), fp);
                 ok += placed;
             }
             const uint64_t fails = __ballot(!placed);
             if (fails != 0) {
-                // one LDS atomic per wave reserves this wave's run in the workgroup buffer
+                // one LDS atomic per wave reserves this wave's run in the overflow buffer
                 const int leader = __ffsll(static_cast<long long>(fails)) - 1;
                 uint32_t base = 0;
                 if (lane == leader) base = atomicAdd(&ovCount, static_cast<uint32_t>(__popcll(fails)));
                 base = __shfl(base, leader, 64);
                 const uint32_t mine = base + static_cast<uint32_t>(__popcll(fails & ltMask));
                 if (!placed) {
+                    const uint64_t entry = ((b0 + local) << 32) | static_cast<uint64_t>(fp);
                     if (mine < kOverflowLds) {
                         ovBuf[mine] = entry;
                     } else {
@@ -219,8 +281,8 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict
                 }
             }
         }
+        __syncthreads();  // the retry list is refilled by the next round
     }
-    __syncthreads();
     const uint32_t buffered = ovCount < kOverflowLds ? ovCount : kOverflowLds;
     if (threadIdx.x == 0 && buffered != 0) ovBase = atomicAdd(overflowCount, static_cast<unsigned long long>(buffered));
     for (size_t i = threadIdx.x; i < vecs; i += kBinBlock) {
@@ -453,7 +515,7 @@ struct BinnedInsert {
             unsigned long long* counters = reinterpret_cast<unsigned long long*>(p);  // overflow, evict, dfs
 
             EntrySource<Config, SortedSemantics> src{keys + off, v.numBuckets};
-            const size_t lds = binLdsBytes((size_t(1) << l2s) * kBucketBytes);
+            const size_t lds = binLdsBytes((size_t(1) << l2s) * kBucketBytes, sizeof(BinRetryOf<Config>));
             if (twoLevel(l2s, sliceBits)) {
                 // a holds the first pass's 32-bit values, b the second's; after
                 // the primary attempts the overflow list goes to a, eviction

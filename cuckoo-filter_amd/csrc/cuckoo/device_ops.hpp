@@ -236,20 +236,17 @@ struct FilterOps {
         }
     }
 
-    /// tryInsertAtBucket on a bucket in LDS: the start word alone first (the
-    /// reference also reads it first, CuckooFilter.cuh:866-872), and the
-    /// whole bucket only when that word is full or its CAS loses.  Reading
-    /// one word instead of the bucket saves LDS reads and the SWAR masks of
-    /// the other words in the common case.
-    static __device__ __forceinline__ bool tryInsertLds(Word* bw, Tag tag) {
+    /// The first step of tryInsertAtBucket on a bucket in LDS: one attempt at
+    /// the tag's start word, where the reference's scan starts
+    /// (CuckooFilter.cuh:866-872).  False if that word is full or the CAS
+    /// lost; the caller then continues with the whole bucket (tryInsertSnap).
+    static __device__ __forceinline__ bool tryStartWordLds(Word* bw, Tag tag) {
         const uint32_t start = startWord(tag);
         const Word cur = bw[start];
         const Word zm = getZeroMask<Tag, Word>(cur);
-        if (zm != 0) {
-            const uint32_t lane = lowestBit(zm) / BITS;
-            if (detail::casRelaxed(bw + start, cur, replaceTag(cur, lane, tag)) == cur) return true;
-        }
-        return tryInsertSnap(bw, detail::loadBucket<Word, WC>(bw), tag);
+        if (zm == 0) return false;
+        const uint32_t lane = lowestBit(zm) / BITS;
+        return detail::casRelaxed(bw + start, cur, replaceTag(cur, lane, tag)) == cur;
     }
 
     /// tryRemoveAtBucket on a snapshot (CuckooFilter.cuh:805-848).
