@@ -9,6 +9,12 @@
 //    (the shape of a lookup's second probe).
 //  * rl_cas: per key, one relaxed device-scope 64-bit CAS at a random word
 //    (the write half of an insert).
+//  * rl_cas_xcd: the same CAS, but each workgroup reads the XCD it runs on
+//    (hardware register XCC_ID) and only touches that XCD's eighth of the
+//    table, so no line is shared between XCDs; with workgroup scope the CAS
+//    can then complete in the XCD's own L2 (agent scope for comparison).
+//  * rl_store_bytes: per key, one byte stored at a random position (the
+//    output of a lookup or delete whose keys were reordered).
 //  * rl_copy: a plain streaming copy, used to calibrate the FETCH_SIZE /
 //    WRITE_SIZE counters against a known byte count.
 #include <hip/hip_runtime.h>
@@ -66,6 +72,32 @@ __global__ __launch_bounds__(1024) void casKernel(unsigned long long* __restrict
     }
 }
 
+/// XCC_ID (hwreg 20, bits [3:0]) of the XCD this wave runs on.
+__device__ __forceinline__ unsigned xccId() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20); }
+
+template <int SCOPE>
+__global__ __launch_bounds__(1024) void casXcdKernel(unsigned long long* __restrict__ words, uint64_t regionMask,
+                                                     int regionShift, unsigned regions,
+                                                     const uint64_t* __restrict__ keys, size_t n) {
+    unsigned long long* base = words + (static_cast<uint64_t>(xccId() % regions) << regionShift);
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i];
+        unsigned long long* p = base + (mix(k) & regionMask);
+        unsigned long long expected = 0;
+        __hip_atomic_compare_exchange_strong(p, &expected, k | 1ull, __ATOMIC_RELAXED, __ATOMIC_RELAXED, SCOPE);
+    }
+}
+
+__global__ __launch_bounds__(1024) void storeBytesKernel(uint8_t* __restrict__ out, uint64_t mask,
+                                                         const uint64_t* __restrict__ keys, size_t n) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i];
+        out[mix(k) & mask] = static_cast<uint8_t>(k);
+    }
+}
+
 __global__ __launch_bounds__(1024) void copyKernel(const V4* __restrict__ src, V4* __restrict__ dst, size_t n16) {
     const size_t stride = size_t(gridDim.x) * blockDim.x;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
@@ -118,6 +150,36 @@ int rl_cas(void* table, size_t table_bytes, const uint64_t* keys, size_t n, int 
     hipStream_t s = static_cast<hipStream_t>(stream);
     casKernel<<<gridFor((const void*)casKernel, n, block), block, 0, s>>>(
         static_cast<unsigned long long*>(table), table_bytes / 8 - 1, keys, n);
+    return static_cast<int>(hipGetLastError());
+}
+
+// regions: the table is split into this many equal parts (a power of two),
+// one per XCD; workgroup_scope selects the CAS scope (else agent).
+int rl_cas_xcd(void* table, size_t table_bytes, unsigned regions, int workgroup_scope, const uint64_t* keys,
+               size_t n, int block, void* stream) {
+    if (!table || !keys || block < 64 || block > 1024 || table_bytes % 8 || !pow2(table_bytes / 8) ||
+        !pow2(regions) || table_bytes / 8 < regions)
+        return -1;
+    const uint64_t perRegion = table_bytes / 8 / regions;
+    int shift = 0;
+    while ((uint64_t(1) << shift) < perRegion) ++shift;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto* w = static_cast<unsigned long long*>(table);
+    if (workgroup_scope) {
+        auto k = casXcdKernel<__HIP_MEMORY_SCOPE_WORKGROUP>;
+        k<<<gridFor((const void*)k, n, block), block, 0, s>>>(w, perRegion - 1, shift, regions, keys, n);
+    } else {
+        auto k = casXcdKernel<__HIP_MEMORY_SCOPE_AGENT>;
+        k<<<gridFor((const void*)k, n, block), block, 0, s>>>(w, perRegion - 1, shift, regions, keys, n);
+    }
+    return static_cast<int>(hipGetLastError());
+}
+
+int rl_store_bytes(void* out, size_t out_bytes, const uint64_t* keys, size_t n, int block, void* stream) {
+    if (!out || !keys || block < 64 || block > 1024 || !pow2(out_bytes)) return -1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    storeBytesKernel<<<gridFor((const void*)storeBytesKernel, n, block), block, 0, s>>>(
+        static_cast<uint8_t*>(out), out_bytes - 1, keys, n);
     return static_cast<int>(hipGetLastError());
 }
 

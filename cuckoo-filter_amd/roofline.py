@@ -7,7 +7,10 @@ Runs the kernels of libcuckoo_roofline.so with HIP events:
   gather128           : random aligned 128-byte reads (one full L2 line);
   gather32nt          : gather32 with non-temporal loads;
   gather32x2 dep      : two dependent 32-byte reads per key (lookup miss shape);
-  cas8                : one random 64-bit CAS per key;
+  cas8                : one random 64-bit CAS per key (device scope);
+  cas8 xcd            : the same, each workgroup confined to the eighth of the
+                        table its XCD owns, at workgroup and at device scope;
+  store1 random       : one byte stored at a random place of a 64 MiB array;
   copy                : streaming copy (HBM bandwidth reference).
 Each figure is algorithmic bytes (the bytes named above) per second.
 
@@ -34,8 +37,10 @@ def _lib():
     P, sz, i = C.c_void_p, C.c_size_t, C.c_int
     L.rl_gather.argtypes = [P, sz, i, P, sz, P, i, i, i, i, P]
     L.rl_cas.argtypes = [P, sz, P, sz, i, P]
+    L.rl_cas_xcd.argtypes = [P, sz, C.c_uint, i, P, sz, i, P]
+    L.rl_store_bytes.argtypes = [P, sz, P, sz, i, P]
     L.rl_copy.argtypes = [P, P, sz, i, P]
-    for f in (L.rl_gather, L.rl_cas, L.rl_copy):
+    for f in (L.rl_gather, L.rl_cas, L.rl_cas_xcd, L.rl_store_bytes, L.rl_copy):
         f.restype = i
     return L
 
@@ -89,9 +94,22 @@ def measure(table_bytes=512 << 20, n=1 << 26, reps=5, block=256, big_factor=4) -
         t = _time(torch, cas, reps)
         res[f"cas8_{tag}_gbs"] = round((8 + 8) * n / t / 1e9, 1)
         res[f"cas8_{tag}_gops"] = round(n / t / 1e9, 2)
+        for wg in (1, 0):
+            def cas_xcd():
+                rc = L.rl_cas_xcd(C.c_void_p(tab.data_ptr()), tb, 8, wg, C.c_void_p(keys.data_ptr()), n, block, s)
+                assert rc == 0, rc
+            t = _time(torch, cas_xcd, reps)
+            res[f"cas8_xcd_{'wg' if wg else 'agent'}_{tag}_gops"] = round(n / t / 1e9, 2)
         del tab
         torch.cuda.empty_cache()
 
+    sink = torch.empty(64 << 20, dtype=torch.uint8, device=dev)
+
+    def store():
+        rc = L.rl_store_bytes(C.c_void_p(sink.data_ptr()), sink.numel(), C.c_void_p(keys.data_ptr()), n, block, s)
+        assert rc == 0, rc
+    res["store1_random_64MiB_gops"] = round(n / _time(torch, store, reps) / 1e9, 2)
+    del sink
     res["copy_gbs"] = measure_copy(reps=reps, block=block)
     return res
 
