@@ -53,12 +53,13 @@ static_assert(kTile % (64 * kWaves) == 0, "tile must split evenly into wave rows
 static_assert(kRowsPerWave == kItems, "each lane keeps one value per row in registers");
 
 /// Reads element i of an array.  Sources split a read into load() (the
-/// memory access) and cook() (any arithmetic on it), so that a kernel can
-/// issue the loads of its next tile before it needs the values.
+/// memory access, base()[i]) and cook() (any arithmetic on it), so that a
+/// kernel can issue the loads of its next tile before it needs the values.
 template <typename U>
 struct ArraySource {
     using Raw = U;
     const U* data;
+    __device__ __forceinline__ const Raw* base() const { return data; }
     __device__ __forceinline__ Raw load(size_t i) const { return data[i]; }
     __device__ __forceinline__ U cook(Raw r) const { return r; }
     __device__ __forceinline__ U operator()(size_t i) const { return data[i]; }
@@ -239,19 +240,29 @@ constexpr size_t scatterLdsBytes() {
     return static_cast<size_t>(kTile) * sizeof(U);
 }
 
+/// Dynamic LDS bytes of the unstable scatters for output values O: the
+/// staged values, then one digit byte per value.
+template <typename O>
+constexpr size_t unstableLdsBytes() {
+    return static_cast<size_t>(kTile) * (sizeof(O) + 1);
+}
+
 /// Steps of an unstable one-tile scatter, for passes whose order inside a
 /// digit does not matter (the first pass of any partition, and the second
 /// pass of the two-level one).  A value's rank among the tile's values of
 /// its digit is the return of one LDS atomic on that digit's counter (kept
 /// in a register), so a value is ranked once, with no ballots.  `vals` holds
-/// this lane's values (row r of wave w is tile positions (w * kRowsPerWave +
-/// r) * 64 + lane), the first `tileCount` valid; `myBase` is, for threads
-/// below kRadix, the global offset of digit threadIdx.x for this tile.
-/// `hist` (kRadix counters) must be zero on entry, with a barrier passed.
-template <typename U, typename P>
-__device__ __forceinline__ void scatterTileUnstable(const U (&vals)[kRowsPerWave], uint32_t tileCount, int shift,
-                                                    uint32_t myBase, P pack, typename P::Out* __restrict__ out,
-                                                    U* stage, uint32_t* hist, uint32_t* runShift) {
+/// this lane's output values (row r of wave w is tile positions (w *
+/// kRowsPerWave + r) * 64 + lane), already transformed, and `digits` their
+/// digits (8 bits per row); the first `tileCount` are valid.  `myBase` is,
+/// for threads below kRadix, the global offset of digit threadIdx.x for this
+/// tile.  `hist` (kRadix counters) must be zero on entry, with a barrier
+/// passed.
+template <typename O>
+__device__ __forceinline__ void scatterTileUnstable(const O (&vals)[kRowsPerWave], uint64_t digits,
+                                                    uint32_t tileCount, uint32_t myBase, O* __restrict__ out,
+                                                    O* stage, uint8_t* dstage, uint32_t* hist, uint32_t* runShift) {
+    static_assert(kRowsPerWave * kDigitBits <= 64, "digits of a lane must pack into 64 bits");
     const int lane = __lane_id();
     const int wave = threadIdx.x / 64;
     const uint32_t waveBase = static_cast<uint32_t>(wave) * kRowsPerWave * 64;
@@ -260,7 +271,7 @@ __device__ __forceinline__ void scatterTileUnstable(const U (&vals)[kRowsPerWave
 #pragma unroll
     for (int r = 0; r < kRowsPerWave; ++r) {
         const bool valid = waveBase + static_cast<uint32_t>(r) * 64 + lane < tileCount;
-        const uint32_t d = static_cast<uint32_t>(vals[r] >> shift) & (kRadix - 1);
+        const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
         const uint32_t k = valid ? atomicAdd(&hist[d], 1u) : 0u;
         if (r % 2 == 0) {
             rank[r / 2] = k;
@@ -278,20 +289,31 @@ __device__ __forceinline__ void scatterTileUnstable(const U (&vals)[kRowsPerWave
         runShift[threadIdx.x] = myBase - start;
     }
     __syncthreads();
-    // 3. stage values digit-ordered in LDS
+    // 3. stage values and their digits digit-ordered in LDS
 #pragma unroll
     for (int r = 0; r < kRowsPerWave; ++r) {
         const bool valid = waveBase + static_cast<uint32_t>(r) * 64 + lane < tileCount;
-        const uint32_t d = static_cast<uint32_t>(vals[r] >> shift) & (kRadix - 1);
+        const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
         const uint32_t k = (rank[r / 2] >> (16 * (r % 2))) & 0xFFFFu;
-        if (valid) stage[hist[d] + k] = vals[r];
+        if (valid) {
+            const uint32_t pos = hist[d] + k;
+            stage[pos] = vals[r];
+            dstage[pos] = static_cast<uint8_t>(d);
+        }
     }
     __syncthreads();
-    // 4. contiguous runs per digit to their global place
-    for (uint32_t j = threadIdx.x; j < tileCount; j += kBlock) {
-        const U v = stage[j];
-        const uint32_t d = static_cast<uint32_t>(v >> shift) & (kRadix - 1);
-        out[static_cast<uint32_t>(runShift[d] + j)] = pack(v);  // exact: the sum is below n < 2^32
+    // 4. contiguous runs per digit to their global place.  A fixed count of
+    //    unconditional stores (past-the-end lanes repeat the tile's last value
+    //    at its own place, an identical write): the compiler can then count
+    //    them, so that a pipelined caller's wait for its prefetched loads does
+    //    not also wait for these stores, and no loop preheader flushes the
+    //    counter.
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const uint32_t j0 = static_cast<uint32_t>(it) * kBlock + threadIdx.x;
+        const uint32_t j = j0 < tileCount ? j0 : tileCount - 1;
+        out[static_cast<uint32_t>(runShift[dstage[j]] + j)] = stage[j];  // exact: the sum is below n < 2^32
+        if (it % 2 == 1) __builtin_amdgcn_sched_barrier(0);  // bounds the registers held in flight
     }
 }
 
@@ -299,14 +321,19 @@ __device__ __forceinline__ void scatterTileUnstable(const U (&vals)[kRowsPerWave
 /// a resident grid in which workgroup b handles tiles b, b + gridDim.x, ...
 /// and issues the loads of its next tile before the current tile is ranked,
 /// staged and written, so the loads overlap the arithmetic and LDS work.  It
-/// is held to 80 VGPRs so that three workgroups (the LDS limit) fit per CU.
+/// is held to 80 VGPRs so that three workgroups fit per CU; at that bound the
+/// compiler spills a few loop invariants, which measured faster (5.94 against
+/// 6.03 ms per 255M-key insert) than 96 VGPRs without spills and two
+/// workgroups per CU.
 template <typename U, typename Src, typename P = Identity<U>>
 __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, typename P::Out* __restrict__ out, size_t n,
                                                            int shift, const uint32_t* __restrict__ offsets,
                                                            uint32_t numTiles, P pack = P{}) {
     using Raw = typename Src::Raw;
+    using O = typename P::Out;
     extern __shared__ __align__(16) unsigned char ldsRaw[];
-    U* stage = reinterpret_cast<U*>(ldsRaw);
+    O* stage = reinterpret_cast<O*>(ldsRaw);
+    uint8_t* dstage = ldsRaw + static_cast<size_t>(kTile) * sizeof(O);
     __shared__ uint32_t hist[kRadix];
     __shared__ uint32_t runShift[kRadix];  // global start of the digit's run minus its start in the tile
 
@@ -317,40 +344,44 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, typename P::
     if (t >= numTiles) return;
     // Loads are unconditional (past-the-end lanes read element n - 1 and are
     // masked out later), so the compiler can count them: a wait for one
-    // value then does not wait for the loads issued after it.
-    const size_t last = n - 1;
-    Raw raw[kRowsPerWave];
+    // value then does not wait for the loads issued after it.  Addresses are
+    // a tile base (uniform, scalar registers) plus a 32-bit lane offset.
+    const uint32_t laneBase = static_cast<uint32_t>(wave) * kRowsPerWave * 64 + lane;
+    auto loadTile = [&](uint32_t tile, Raw (&dst)[kRowsPerWave]) {
+        const size_t tb = static_cast<size_t>(tile) * kTile;
+        const Raw* tp = src.base() + tb;
+        const uint32_t lastLocal = static_cast<uint32_t>(n - 1 - tb < kTile - 1 ? n - 1 - tb : kTile - 1);
 #pragma unroll
-    for (int r = 0; r < kRowsPerWave; ++r) {
-        const size_t idx = static_cast<size_t>(t) * kTile + static_cast<size_t>(wave) * kRowsPerWave * 64 +
-                           static_cast<size_t>(r) * 64 + lane;
-        raw[r] = src.load(idx < last ? idx : last);
-    }
+        for (int r = 0; r < kRowsPerWave; ++r) {
+            const uint32_t li = laneBase + static_cast<uint32_t>(r) * 64;
+            dst[r] = tp[(li < lastLocal ? li : lastLocal) & (kTile - 1)];
+        }
+    };
+    Raw raw[kRowsPerWave];
+    loadTile(t, raw);
     for (; t < numTiles; t += gridDim.x) {
         const size_t tileBase = static_cast<size_t>(t) * kTile;
         const uint32_t tileCount =
             static_cast<uint32_t>(n - tileBase < static_cast<size_t>(kTile) ? n - tileBase : static_cast<size_t>(kTile));
         if (threadIdx.x < kRadix) hist[threadIdx.x] = 0;
-        // this tile's values; then the next tile's loads go out
-        U vals[kRowsPerWave];
+        // this tile's output values and digits; then the next tile's loads go out
+        O vals[kRowsPerWave];
+        uint64_t digits = 0;
 #pragma unroll
-        for (int r = 0; r < kRowsPerWave; ++r) vals[r] = src.cook(raw[r]);
+        for (int r = 0; r < kRowsPerWave; ++r) {
+            const U c = src.cook(raw[r]);
+            digits |= static_cast<uint64_t>(static_cast<uint32_t>(c >> shift) & (kRadix - 1)) << (r * kDigitBits);
+            vals[r] = pack(c);
+            if (r % 2 == 1) __builtin_amdgcn_sched_barrier(0);  // bounds the hashes in flight
+        }
         // this tile's digit bases, loaded before the prefetch so that waiting
         // for them does not wait for the next tile's loads
         const uint32_t myBase =
             offsets[static_cast<size_t>(t) * kRadix + (threadIdx.x < kRadix ? threadIdx.x : kRadix - 1)];
-        {
-            // (past the last tile this reloads the current one; those values go unused)
-            const uint32_t tn = t + gridDim.x < numTiles ? t + gridDim.x : t;
-#pragma unroll
-            for (int r = 0; r < kRowsPerWave; ++r) {
-                const size_t idx = static_cast<size_t>(tn) * kTile +
-                                   static_cast<size_t>(wave) * kRowsPerWave * 64 + static_cast<size_t>(r) * 64 + lane;
-                raw[r] = src.load(idx < last ? idx : last);
-            }
-        }
+        // (past the last tile this reloads the current one; those values go unused)
+        loadTile(t + gridDim.x < numTiles ? t + gridDim.x : t, raw);
         __syncthreads();  // hist cleared
-        scatterTileUnstable<U, P>(vals, tileCount, shift, myBase, pack, out, stage, hist, runShift);
+        scatterTileUnstable<O>(vals, digits, tileCount, myBase, out, stage, dstage, hist, runShift);
         __syncthreads();  // stage, hist and the bases are reused by the next tile
     }
 }
@@ -568,8 +599,10 @@ __global__ __launch_bounds__(kBlock) void scatterSegKernel(const U* __restrict__
                                                            const uint32_t* __restrict__ partStart,
                                                            const uint32_t* __restrict__ tileBase, int shift,
                                                            const uint32_t* __restrict__ offsets, P pack) {
+    using O = typename P::Out;
     extern __shared__ __align__(16) unsigned char ldsRaw[];
-    U* stage = reinterpret_cast<U*>(ldsRaw);
+    O* stage = reinterpret_cast<O*>(ldsRaw);
+    uint8_t* dstage = ldsRaw + static_cast<size_t>(kTile) * sizeof(O);
     __shared__ uint32_t hist[kRadix];
     __shared__ uint32_t runShift[kRadix];
     uint32_t p, lo, hi;
@@ -579,16 +612,19 @@ __global__ __launch_bounds__(kBlock) void scatterSegKernel(const U* __restrict__
     const int wave = threadIdx.x / 64;
     const uint32_t waveBase = lo + static_cast<uint32_t>(wave) * kRowsPerWave * 64;
     const uint32_t last = hi - 1;
-    U vals[kRowsPerWave];
+    O vals[kRowsPerWave];
+    uint64_t digits = 0;
 #pragma unroll
     for (int r = 0; r < kRowsPerWave; ++r) {
         const uint32_t idx = waveBase + static_cast<uint32_t>(r) * 64 + lane;
-        vals[r] = in[idx < last ? idx : last];  // past-the-end lanes are masked by tileCount
+        const U v = in[idx < last ? idx : last];  // past-the-end lanes are masked by tileCount
+        digits |= static_cast<uint64_t>(static_cast<uint32_t>(v >> shift) & (kRadix - 1)) << (r * kDigitBits);
+        vals[r] = pack(v);
     }
     const uint32_t myBase =
         offsets[static_cast<size_t>(blockIdx.x) * kRadix + (threadIdx.x < kRadix ? threadIdx.x : kRadix - 1)];
     __syncthreads();  // hist cleared
-    scatterTileUnstable<U, P>(vals, hi - lo, shift, myBase, pack, out, stage, hist, runShift);
+    scatterTileUnstable<O>(vals, digits, hi - lo, myBase, out, stage, dstage, hist, runShift);
 }
 
 /// Grid of the pipelined scatter kernel: its resident workgroups on the
@@ -601,7 +637,7 @@ inline uint32_t scatterGrid(uint32_t numTiles) {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, scatterKernel<U, Src, P>, kBlock,
-                                                     scatterLdsBytes<U>()) !=
+                                                     unstableLdsBytes<typename P::Out>()) !=
             hipSuccess ||
         cus <= 0 || perCu <= 0) {
         return numTiles;
@@ -663,12 +699,13 @@ inline hipError_t partitionTwoLevel(Src first, size_t n, int shift1, P1 pack1, i
     uint32_t* tileBase = partStart + kRadix + 1;
     countKernel<U, Src><<<tiles, kBlock, 0, stream>>>(first, n, shift1, counts, tiles);
     scanCounts(counts, tiles, chunkSum, partial, stream);
-    scatterKernel<U, Src, P1><<<scatterGrid<U, Src, P1>(tiles), kBlock, scatterLdsBytes<U>(), stream>>>(
+    scatterKernel<U, Src, P1><<<scatterGrid<U, Src, P1>(tiles), kBlock, unstableLdsBytes<V>(), stream>>>(
         first, a, n, shift1, counts, tiles, pack1);
     segmentKernel<uint32_t><<<1, kRadix, 0, stream>>>(counts, static_cast<uint32_t>(n), partStart, tileBase);
     countSegKernel<V><<<tiles2, kBlock, 0, stream>>>(a, partStart, tileBase, shift2, radix2, counts);
     segScanKernel<uint32_t><<<kRadix, kRadix, 0, stream>>>(counts, partStart, tileBase, radix2, groupStart, numGroups);
-    scatterSegKernel<V, P2><<<tiles2, kBlock, scatterLdsBytes<V>(), stream>>>(a, b, partStart, tileBase, shift2,
+    scatterSegKernel<V, P2><<<tiles2, kBlock, unstableLdsBytes<typename P2::Out>(), stream>>>(a, b, partStart,
+                                                                                               tileBase, shift2,
                                                                                counts, pack2);
     return hipGetLastError();
 }
@@ -700,7 +737,7 @@ inline hipError_t sortBits(Src first, size_t n, int lo, int hi, U* a, U* b, void
         }
         scanCounts(counts, tiles, chunkSum, partial, stream);
         if (firstPass) {
-            scatterKernel<U, Src><<<scatterGrid<U, Src>(tiles), kBlock, scatterLdsBytes<U>(), stream>>>(
+            scatterKernel<U, Src><<<scatterGrid<U, Src>(tiles), kBlock, unstableLdsBytes<U>(), stream>>>(
                 first, dst, n, shift, counts, tiles);
         } else {
             scatterTileKernel<U, ArraySource<U>><<<tiles, kBlock, scatterLdsBytes<U>(), stream>>>(
