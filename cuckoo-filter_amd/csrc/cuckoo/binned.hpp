@@ -199,6 +199,19 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict
         retryCount[0] = 0;
         retryCount[1] = 0;
     }
+    // A round's entries are loaded one round ahead (kBinDepth per thread, each
+    // load coalesced across the wave; past-the-end lanes reread the last), the
+    // first round's together with the slice.
+    const uint32_t last = hi - 1;
+    E next[kBinDepth];
+    auto loadRound = [&](uint32_t r0) {
+#pragma unroll
+        for (uint32_t k = 0; k < kBinDepth; ++k) {
+            const uint32_t e = r0 + k * kBinBlock + threadIdx.x;
+            next[k] = entries[e < last ? e : last];
+        }
+    };
+    loadRound(lo);
     using V4 = unsigned int __attribute__((ext_vector_type(4)));
     const size_t bytes = words * sizeof(Word);
     const size_t vecs = bytes / 16;
@@ -211,17 +224,13 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict
     uint32_t ok = 0;
     const int lane = __lane_id();
     const uint64_t ltMask = (1ull << lane) - 1;
-    const uint32_t last = hi - 1;
     uint32_t parity = 0;
     for (uint32_t r0 = lo; r0 < hi; r0 += kBinRound, parity ^= 1u) {
-        // 1. start words; kBinDepth entries per thread, loaded together (each
-        //    load coalesced across the wave; past-the-end lanes reread the last)
+        // 1. start words
         E ent[kBinDepth];
 #pragma unroll
-        for (uint32_t k = 0; k < kBinDepth; ++k) {
-            const uint32_t e = r0 + k * kBinBlock + threadIdx.x;
-            ent[k] = entries[e < last ? e : last];
-        }
+        for (uint32_t k = 0; k < kBinDepth; ++k) ent[k] = next[k];
+        loadRound(r0 + kBinRound < hi ? r0 + kBinRound : r0);  // (past the last round: a harmless reload)
 #pragma unroll
         for (uint32_t k = 0; k < kBinDepth; ++k) {
             const bool valid = r0 + k * kBinBlock + threadIdx.x < hi;
