@@ -53,8 +53,8 @@ struct EntrySource {
     const Key* keys;
     size_t numBuckets;
 
-    __device__ __forceinline__ const Raw* base() const { return keys; }
     __device__ __forceinline__ Raw load(size_t i) const { return keys[i]; }
+    __device__ __forceinline__ Raw loadRel(size_t tileBase, uint32_t off) const { return (keys + tileBase)[off]; }
     __device__ __forceinline__ uint64_t operator()(size_t i) const { return cook(keys[i]); }
     __device__ __forceinline__ uint64_t cook(Key key) const {
         const auto c = Config::AltBucketPolicy::getCandidateBuckets(key, numBuckets);

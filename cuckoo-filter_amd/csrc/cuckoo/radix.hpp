@@ -59,8 +59,9 @@ template <typename U>
 struct ArraySource {
     using Raw = U;
     const U* data;
-    __device__ __forceinline__ const Raw* base() const { return data; }
     __device__ __forceinline__ Raw load(size_t i) const { return data[i]; }
+    /// load(tileBase + off) with a uniform tileBase and a 32-bit offset
+    __device__ __forceinline__ Raw loadRel(size_t tileBase, uint32_t off) const { return (data + tileBase)[off]; }
     __device__ __forceinline__ U cook(Raw r) const { return r; }
     __device__ __forceinline__ U operator()(size_t i) const { return data[i]; }
 };
@@ -349,12 +350,11 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, typename P::
     const uint32_t laneBase = static_cast<uint32_t>(wave) * kRowsPerWave * 64 + lane;
     auto loadTile = [&](uint32_t tile, Raw (&dst)[kRowsPerWave]) {
         const size_t tb = static_cast<size_t>(tile) * kTile;
-        const Raw* tp = src.base() + tb;
         const uint32_t lastLocal = static_cast<uint32_t>(n - 1 - tb < kTile - 1 ? n - 1 - tb : kTile - 1);
 #pragma unroll
         for (int r = 0; r < kRowsPerWave; ++r) {
             const uint32_t li = laneBase + static_cast<uint32_t>(r) * 64;
-            dst[r] = tp[(li < lastLocal ? li : lastLocal) & (kTile - 1)];
+            dst[r] = src.loadRel(tb, (li < lastLocal ? li : lastLocal) & (kTile - 1));
         }
     };
     Raw raw[kRowsPerWave];
@@ -383,6 +383,147 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, typename P::
         __syncthreads();  // hist cleared
         scatterTileUnstable<O>(vals, digits, tileCount, myBase, out, stage, dstage, hist, runShift);
         __syncthreads();  // stage, hist and the bases are reused by the next tile
+    }
+}
+
+/// Values per tile of the wide first pass: two halves of kTile, so that a
+/// digit's run in the output is twice as long (with 256 digits, 32 values
+/// instead of 16: fewer partial-line writes) while a lane still holds one
+/// half's values at a time.
+inline constexpr int kWideTile = 2 * kTile;
+
+/// Digit histogram per wide tile (tile-major rows, as countKernel).
+template <typename U, typename Src>
+__global__ __launch_bounds__(kBlock) void countWideKernel(Src src, size_t n, int shift, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t hist[kRadix];
+    for (int i = threadIdx.x; i < kRadix; i += kBlock) hist[i] = 0;
+    __syncthreads();
+    const size_t base = static_cast<size_t>(blockIdx.x) * kWideTile;
+    const size_t last = n - 1;
+    typename Src::Raw raw[2 * kItems];
+#pragma unroll
+    for (int k = 0; k < 2 * kItems; ++k) {
+        const size_t idx = base + static_cast<size_t>(k) * kBlock + threadIdx.x;
+        raw[k] = src.load(idx < last ? idx : last);
+    }
+#pragma unroll
+    for (int k = 0; k < 2 * kItems; ++k) {
+        const size_t idx = base + static_cast<size_t>(k) * kBlock + threadIdx.x;
+        const uint32_t d = static_cast<uint32_t>(static_cast<U>(src.cook(raw[k])) >> shift) & (kRadix - 1);
+        if (idx < n) atomicAdd(&hist[d], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kRadix; i += kBlock) counts[static_cast<size_t>(blockIdx.x) * kRadix + i] = hist[i];
+}
+
+/// Dynamic LDS of scatterWideKernel for output values O.
+template <typename O>
+constexpr size_t wideLdsBytes() {
+    return static_cast<size_t>(kWideTile) * (sizeof(O) + 1);
+}
+
+/// The pipelined first-pass scatter over wide tiles (unstable).  A tile's
+/// digit counts are differences of the scanned offset matrix (the offset of
+/// (digit, tile + 1), or of (digit + 1, tile 0) after the last tile, minus
+/// that of (digit, tile)), so each digit's place in the tile's staging area
+/// is known before any value is ranked: a value's staging position is the
+/// return of one LDS atomic on its digit's cursor, and each half of the tile
+/// is staged as soon as it is computed.  The loads of the next half (of this
+/// tile, then of the next tile) go out before the current half is staged.
+template <typename U, typename Src, typename P>
+__global__ __launch_bounds__(kBlock, 6) void scatterWideKernel(Src src, typename P::Out* __restrict__ out, size_t n,
+                                                               int shift, const uint32_t* __restrict__ offsets,
+                                                               uint32_t numTiles, P pack) {
+    using Raw = typename Src::Raw;
+    using O = typename P::Out;
+    extern __shared__ __align__(16) unsigned char ldsRaw[];
+    O* stage = reinterpret_cast<O*>(ldsRaw);
+    uint8_t* dstage = ldsRaw + static_cast<size_t>(kWideTile) * sizeof(O);
+    __shared__ uint32_t cursor[kRadix];
+    __shared__ uint32_t runShift[kRadix];
+
+    uint32_t t = blockIdx.x;
+    if (t >= numTiles) return;
+    const int lane = __lane_id();
+    const int wave = threadIdx.x / 64;
+    const uint32_t laneBase = static_cast<uint32_t>(wave) * kRowsPerWave * 64 + lane;
+    // Loads are unconditional: past-the-end lanes (and a whole half past the
+    // end) read element n - 1, and are masked out when staged.
+    auto loadHalf = [&](uint32_t tile, int half, Raw (&dst)[kRowsPerWave]) {
+        const size_t hb0 = static_cast<size_t>(tile) * kWideTile + static_cast<size_t>(half) * kTile;
+        const size_t hb = hb0 < n ? hb0 : n - 1;
+        const uint32_t lastLocal = static_cast<uint32_t>(n - 1 - hb < kTile - 1 ? n - 1 - hb : kTile - 1);
+#pragma unroll
+        for (int r = 0; r < kRowsPerWave; ++r) {
+            const uint32_t li = laneBase + static_cast<uint32_t>(r) * 64;
+            dst[r] = src.loadRel(hb, (li < lastLocal ? li : lastLocal) & (kTile - 1));
+        }
+    };
+    // a half's output values and digits (8 bits per row), from the loaded raw values
+    O vals[kRowsPerWave];
+    uint64_t digits = 0;
+    auto cookHalf = [&](const Raw (&in)[kRowsPerWave]) {
+        digits = 0;
+#pragma unroll
+        for (int r = 0; r < kRowsPerWave; ++r) {
+            const U c = src.cook(in[r]);
+            digits |= static_cast<uint64_t>(static_cast<uint32_t>(c >> shift) & (kRadix - 1)) << (r * kDigitBits);
+            vals[r] = pack(c);
+        }
+    };
+    // stages a half: each valid value at its digit's cursor
+    auto stageHalf = [&](size_t halfBase) {
+#pragma unroll
+        for (int r = 0; r < kRowsPerWave; ++r) {
+            const uint32_t d = static_cast<uint32_t>(digits >> (r * kDigitBits)) & (kRadix - 1);
+            if (halfBase + laneBase + static_cast<uint32_t>(r) * 64 < n) {
+                const uint32_t pos = atomicAdd(&cursor[d], 1u);
+                stage[pos] = vals[r];
+                dstage[pos] = static_cast<uint8_t>(d);
+            }
+        }
+    };
+    Raw raw[kRowsPerWave];
+    loadHalf(t, 0, raw);
+    for (; t < numTiles; t += gridDim.x) {
+        const size_t tileBase = static_cast<size_t>(t) * kWideTile;
+        const uint32_t tileCount = static_cast<uint32_t>(
+            n - tileBase < static_cast<size_t>(kWideTile) ? n - tileBase : static_cast<size_t>(kWideTile));
+        // this tile's digit counts and bases
+        uint32_t myBase = 0, cnt = 0;
+        if (threadIdx.x < kRadix) {
+            const uint32_t d = threadIdx.x;
+            myBase = offsets[static_cast<size_t>(t) * kRadix + d];
+            const uint32_t next = t + 1 < numTiles ? offsets[static_cast<size_t>(t + 1) * kRadix + d]
+                                                   : (d + 1 < kRadix ? offsets[d + 1] : static_cast<uint32_t>(n));
+            cnt = next - myBase;
+        }
+        uint32_t total;
+        const uint32_t start = blockExclusiveScan<kBlock>(cnt, total);
+        if (threadIdx.x < kRadix) {
+            cursor[threadIdx.x] = start;
+            runShift[threadIdx.x] = myBase - start;
+        }
+        __syncthreads();  // cursors set
+        // first half (already loaded); the second half's loads go out before it is staged
+        cookHalf(raw);
+        loadHalf(t, 1, raw);
+        stageHalf(tileBase);
+        // second half; the next tile's first half goes out before it is staged
+        cookHalf(raw);
+        loadHalf(t + gridDim.x < numTiles ? t + gridDim.x : t, 0, raw);
+        stageHalf(tileBase + kTile);
+        __syncthreads();
+        // contiguous runs per digit to their global place: a fixed count of
+        // unconditional stores (past-the-end lanes repeat the last value)
+#pragma unroll
+        for (int it = 0; it < 2 * kItems; ++it) {
+            const uint32_t j0 = static_cast<uint32_t>(it) * kBlock + threadIdx.x;
+            const uint32_t j = j0 < tileCount ? j0 : tileCount - 1;
+            out[static_cast<uint32_t>(runShift[dstage[j]] + j)] = stage[j];  // exact: below n < 2^32
+            if (it % 2 == 1) __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();  // staging area and cursors are reused by the next tile
     }
 }
 
@@ -646,6 +787,23 @@ inline uint32_t scatterGrid(uint32_t numTiles) {
     return numTiles < resident ? numTiles : resident;
 }
 
+/// Grid of the wide first-pass kernel: its resident workgroups (at most one per tile).
+template <typename U, typename Src, typename P>
+inline uint32_t wideGrid(uint32_t numTiles) {
+    int dev = 0;
+    int cus = 0;
+    int perCu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, scatterWideKernel<U, Src, P>, kBlock,
+                                                     wideLdsBytes<typename P::Out>()) != hipSuccess ||
+        cus <= 0 || perCu <= 0) {
+        return numTiles;
+    }
+    const uint32_t resident = static_cast<uint32_t>(cus) * static_cast<uint32_t>(perCu);
+    return numTiles < resident ? numTiles : resident;
+}
+
 /// Scratch needed to partition n values: the two value buffers are the
 /// caller's; this is the count/scan space in bytes.
 inline size_t scratchBytes(size_t n) {
@@ -697,10 +855,11 @@ inline hipError_t partitionTwoLevel(Src first, size_t n, int shift1, P1 pack1, i
     uint32_t* partial = chunkSum + static_cast<size_t>(chunks) * kRadix;
     uint32_t* partStart = partial + (static_cast<size_t>(chunks) * kRadix + kScanChunk - 1) / kScanChunk;
     uint32_t* tileBase = partStart + kRadix + 1;
-    countKernel<U, Src><<<tiles, kBlock, 0, stream>>>(first, n, shift1, counts, tiles);
-    scanCounts(counts, tiles, chunkSum, partial, stream);
-    scatterKernel<U, Src, P1><<<scatterGrid<U, Src, P1>(tiles), kBlock, unstableLdsBytes<V>(), stream>>>(
-        first, a, n, shift1, counts, tiles, pack1);
+    const uint32_t wideTiles = static_cast<uint32_t>((n + kWideTile - 1) / kWideTile);
+    countWideKernel<U, Src><<<wideTiles, kBlock, 0, stream>>>(first, n, shift1, counts);
+    scanCounts(counts, wideTiles, chunkSum, partial, stream);
+    scatterWideKernel<U, Src, P1><<<wideGrid<U, Src, P1>(wideTiles), kBlock, wideLdsBytes<V>(), stream>>>(
+        first, a, n, shift1, counts, wideTiles, pack1);
     segmentKernel<uint32_t><<<1, kRadix, 0, stream>>>(counts, static_cast<uint32_t>(n), partStart, tileBase);
     countSegKernel<V><<<tiles2, kBlock, 0, stream>>>(a, partStart, tileBase, shift2, radix2, counts);
     segScanKernel<uint32_t><<<kRadix, kRadix, 0, stream>>>(counts, partStart, tileBase, radix2, groupStart, numGroups);
