@@ -386,28 +386,36 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, typename P::
     }
 }
 
-/// Values per tile of the wide first pass: two halves of kTile, so that a
-/// digit's run in the output is twice as long (with 256 digits, 32 values
-/// instead of 16: fewer partial-line writes) while a lane still holds one
-/// half's values at a time.
-inline constexpr int kWideTile = 2 * kTile;
+/// Values per tile of the wide first pass: PARTS parts of kTile, so that a
+/// digit's run in the output is PARTS times as long (with 256 digits, 16 *
+/// PARTS values: fewer partial-line writes) while a lane still holds one
+/// part's values at a time.
+template <int PARTS>
+inline constexpr int kWideTileOf = PARTS * kTile;
+
+/// Parts per wide tile in the two-level partition's first pass: three
+/// (48-value runs, 60 KiB of LDS, two workgroups per CU) measured faster than
+/// two (32-value runs, three workgroups per CU): 5.53 against 5.63 ms per
+/// 255M-key insert.  Four would leave one workgroup per CU.
+inline constexpr int kWideParts = 3;
 
 /// Digit histogram per wide tile (tile-major rows, as countKernel).
-template <typename U, typename Src>
+template <typename U, typename Src, int PARTS>
 __global__ __launch_bounds__(kBlock) void countWideKernel(Src src, size_t n, int shift, uint32_t* __restrict__ counts) {
+    constexpr int kWideTile = kWideTileOf<PARTS>;
     __shared__ uint32_t hist[kRadix];
     for (int i = threadIdx.x; i < kRadix; i += kBlock) hist[i] = 0;
     __syncthreads();
     const size_t base = static_cast<size_t>(blockIdx.x) * kWideTile;
     const size_t last = n - 1;
-    typename Src::Raw raw[2 * kItems];
+    typename Src::Raw raw[PARTS * kItems];
 #pragma unroll
-    for (int k = 0; k < 2 * kItems; ++k) {
+    for (int k = 0; k < PARTS * kItems; ++k) {
         const size_t idx = base + static_cast<size_t>(k) * kBlock + threadIdx.x;
         raw[k] = src.load(idx < last ? idx : last);
     }
 #pragma unroll
-    for (int k = 0; k < 2 * kItems; ++k) {
+    for (int k = 0; k < PARTS * kItems; ++k) {
         const size_t idx = base + static_cast<size_t>(k) * kBlock + threadIdx.x;
         const uint32_t d = static_cast<uint32_t>(static_cast<U>(src.cook(raw[k])) >> shift) & (kRadix - 1);
         if (idx < n) atomicAdd(&hist[d], 1u);
@@ -417,9 +425,9 @@ __global__ __launch_bounds__(kBlock) void countWideKernel(Src src, size_t n, int
 }
 
 /// Dynamic LDS of scatterWideKernel for output values O.
-template <typename O>
+template <typename O, int PARTS>
 constexpr size_t wideLdsBytes() {
-    return static_cast<size_t>(kWideTile) * (sizeof(O) + 1);
+    return static_cast<size_t>(kWideTileOf<PARTS>) * (sizeof(O) + 1);
 }
 
 /// The pipelined first-pass scatter over wide tiles (unstable).  A tile's
@@ -427,13 +435,14 @@ constexpr size_t wideLdsBytes() {
 /// (digit, tile + 1), or of (digit + 1, tile 0) after the last tile, minus
 /// that of (digit, tile)), so each digit's place in the tile's staging area
 /// is known before any value is ranked: a value's staging position is the
-/// return of one LDS atomic on its digit's cursor, and each half of the tile
-/// is staged as soon as it is computed.  The loads of the next half (of this
-/// tile, then of the next tile) go out before the current half is staged.
-template <typename U, typename Src, typename P>
+/// return of one LDS atomic on its digit's cursor, and each part of the tile
+/// is staged as soon as it is computed.  The loads of the next part (of this
+/// tile, then of the next tile) go out before the current part is staged.
+template <typename U, typename Src, typename P, int PARTS>
 __global__ __launch_bounds__(kBlock, 6) void scatterWideKernel(Src src, typename P::Out* __restrict__ out, size_t n,
                                                                int shift, const uint32_t* __restrict__ offsets,
                                                                uint32_t numTiles, P pack) {
+    constexpr int kWideTile = kWideTileOf<PARTS>;
     using Raw = typename Src::Raw;
     using O = typename P::Out;
     extern __shared__ __align__(16) unsigned char ldsRaw[];
@@ -505,19 +514,22 @@ __global__ __launch_bounds__(kBlock, 6) void scatterWideKernel(Src src, typename
             runShift[threadIdx.x] = myBase - start;
         }
         __syncthreads();  // cursors set
-        // first half (already loaded); the second half's loads go out before it is staged
-        cookHalf(raw);
-        loadHalf(t, 1, raw);
-        stageHalf(tileBase);
-        // second half; the next tile's first half goes out before it is staged
-        cookHalf(raw);
-        loadHalf(t + gridDim.x < numTiles ? t + gridDim.x : t, 0, raw);
-        stageHalf(tileBase + kTile);
+        // each part (the first already loaded); the next part's loads go out before it is staged
+#pragma unroll
+        for (int h = 0; h < PARTS; ++h) {
+            cookHalf(raw);
+            if (h + 1 < PARTS) {
+                loadHalf(t, h + 1, raw);
+            } else {
+                loadHalf(t + gridDim.x < numTiles ? t + gridDim.x : t, 0, raw);
+            }
+            stageHalf(tileBase + static_cast<size_t>(h) * kTile);
+        }
         __syncthreads();
         // contiguous runs per digit to their global place: a fixed count of
         // unconditional stores (past-the-end lanes repeat the last value)
 #pragma unroll
-        for (int it = 0; it < 2 * kItems; ++it) {
+        for (int it = 0; it < PARTS * kItems; ++it) {
             const uint32_t j0 = static_cast<uint32_t>(it) * kBlock + threadIdx.x;
             const uint32_t j = j0 < tileCount ? j0 : tileCount - 1;
             out[static_cast<uint32_t>(runShift[dstage[j]] + j)] = stage[j];  // exact: below n < 2^32
@@ -788,15 +800,15 @@ inline uint32_t scatterGrid(uint32_t numTiles) {
 }
 
 /// Grid of the wide first-pass kernel: its resident workgroups (at most one per tile).
-template <typename U, typename Src, typename P>
+template <typename U, typename Src, typename P, int PARTS>
 inline uint32_t wideGrid(uint32_t numTiles) {
     int dev = 0;
     int cus = 0;
     int perCu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, scatterWideKernel<U, Src, P>, kBlock,
-                                                     wideLdsBytes<typename P::Out>()) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, scatterWideKernel<U, Src, P, PARTS>, kBlock,
+                                                     wideLdsBytes<typename P::Out, PARTS>()) != hipSuccess ||
         cus <= 0 || perCu <= 0) {
         return numTiles;
     }
@@ -855,11 +867,13 @@ inline hipError_t partitionTwoLevel(Src first, size_t n, int shift1, P1 pack1, i
     uint32_t* partial = chunkSum + static_cast<size_t>(chunks) * kRadix;
     uint32_t* partStart = partial + (static_cast<size_t>(chunks) * kRadix + kScanChunk - 1) / kScanChunk;
     uint32_t* tileBase = partStart + kRadix + 1;
-    const uint32_t wideTiles = static_cast<uint32_t>((n + kWideTile - 1) / kWideTile);
-    countWideKernel<U, Src><<<wideTiles, kBlock, 0, stream>>>(first, n, shift1, counts);
+    constexpr int PARTS = kWideParts;
+    const uint32_t wideTiles = static_cast<uint32_t>((n + kWideTileOf<PARTS> - 1) / kWideTileOf<PARTS>);
+    countWideKernel<U, Src, PARTS><<<wideTiles, kBlock, 0, stream>>>(first, n, shift1, counts);
     scanCounts(counts, wideTiles, chunkSum, partial, stream);
-    scatterWideKernel<U, Src, P1><<<wideGrid<U, Src, P1>(wideTiles), kBlock, wideLdsBytes<V>(), stream>>>(
-        first, a, n, shift1, counts, wideTiles, pack1);
+    scatterWideKernel<U, Src, P1, PARTS>
+        <<<wideGrid<U, Src, P1, PARTS>(wideTiles), kBlock, wideLdsBytes<V, PARTS>(), stream>>>(
+            first, a, n, shift1, counts, wideTiles, pack1);
     segmentKernel<uint32_t><<<1, kRadix, 0, stream>>>(counts, static_cast<uint32_t>(n), partStart, tileBase);
     countSegKernel<V><<<tiles2, kBlock, 0, stream>>>(a, partStart, tileBase, shift2, radix2, counts);
     segScanKernel<uint32_t><<<kRadix, kRadix, 0, stream>>>(counts, partStart, tileBase, radix2, groupStart, numGroups);
