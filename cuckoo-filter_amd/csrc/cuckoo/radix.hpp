@@ -28,7 +28,8 @@
 // each partition is split by the second digit on its own, so the bits of
 // the first digit can be dropped from the values (a scatter may write its
 // values through an output transform), and the group starts come out of
-// the second pass's scan.
+// the second pass's scan.  Its first pass works on wide tiles (kWideParts
+// tiles of kTile values) for longer output runs per digit.
 #pragma once
 
 #include <hip/hip_runtime.h>

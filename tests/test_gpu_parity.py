@@ -333,6 +333,42 @@ def test_partition_levels_insert_invariants(cf, oracle_mod, kw, cap, mode, parti
     f.close()
 
 
+@pytest.mark.parametrize("n", [1, 7, 4095, 4097, 12287, 12289, 40961, 100003])
+@pytest.mark.parametrize("kw", [dict(eviction="bfs"), dict(eviction="bfs", bits_per_tag=8, word_bytes=4)],
+                         ids=["t16", "t8"])
+def test_two_level_partition_small_and_ragged_batches(cf, oracle_mod, kw, n, monkeypatch):
+    """Batches far smaller than the table through the two-level partition
+    (forced binned path, 2^24 slots: 1024 slices or more): single keys,
+    sizes around the 4096-value tile and the 12288-value first-pass tile, and
+    partitions that stay empty.  At this load nothing fails, every stored tag
+    sits in one of its key's two buckets, and every key is found, then
+    deleted."""
+    import torch
+
+    monkeypatch.setenv("CUCKOO_INSERT_PATH", "binned")
+    O = oracle_mod
+    c = cf.make_config(**kw)
+    oc = _oracle_cfg(O, c)
+    f = cf.CuckooFilter(1 << 24, config=c)
+    nb = f.getNumBuckets()
+    host = O.gen_keys(900 + n, n)
+    keys = _dev_keys(cf, host)
+    occ = f.insertMany(keys)
+    assert occ == n
+    table = assert_table_count(f, occ, n)
+    _check_invariants(O, oc, f.exportTable(), host, table, nb)
+    assert f.containsMany(keys).all().item()
+    # a second batch on top (the partition's scratch is reused)
+    host2 = O.gen_keys(1900 + n, n)
+    keys2 = _dev_keys(cf, host2)
+    assert f.insertMany(keys2) == 2 * n
+    assert f.containsMany(keys2).all().item() and f.containsMany(keys).all().item()
+    out = torch.zeros(n, dtype=torch.bool, device=keys.device)
+    assert f.deleteMany(keys, out=out) == n and out.all().item()
+    assert f.containsMany(keys2).all().item()
+    f.close()
+
+
 @pytest.mark.parametrize("path", ["direct", "binned"])
 def test_eviction_histogram_accounts_for_every_insert(cf, oracle_mod, path, monkeypatch):
     monkeypatch.setenv("CUCKOO_INSERT_PATH", path)
