@@ -18,7 +18,7 @@ oracle at the same load trajectory (BYTES_PER_KEY below), and `traffic` is the
 HBM bytes per launch read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
 summary in profiles/hbm_traffic_2p28.md.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-load-sweep]
 """
 from __future__ import annotations
 
@@ -115,6 +115,34 @@ def cpu_baseline(n_ins, n_look, n_del, budget_s=25.0):
     t_del = time.perf_counter() - t0
     keys_done = n_ins + nq + nd
     total = t_ins + t_look + t_del
+    del f
+    # the CPU-runnable configuration: capacity 2^20 (2^16 buckets), 996,147
+    # inserts (95%), 2^20 lookups (50/50), then half of the inserted keys
+    # deleted; the sequential restatement and the threaded one
+    c1 = {}
+    for label, thr in (("sequential", 1), (f"{threads}_threads", threads)):
+        g = oracle.Filter(cfg, 1 << 20)
+        n1 = int(g.capacity() * 0.95)
+        k1 = oracle.gen_keys(SEED + 1, n1 + (1 << 19))
+        q1 = np.concatenate([k1[: 1 << 19], k1[n1:]])
+        t0 = time.perf_counter()
+        g.insert_many(k1[:n1], threads=thr)
+        ti = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g.contains_many(q1, threads=thr)
+        tl = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g.delete_many(k1[: n1 // 2], threads=thr)
+        td = time.perf_counter() - t0
+        c1[label] = {"insert_mkeys_s": round(n1 / ti / 1e6, 2), "lookup_mkeys_s": round(q1.size / tl / 1e6, 2),
+                     "delete_mkeys_s": round((n1 // 2) / td / 1e6, 2),
+                     "mkeys_s": round((n1 + q1.size + n1 // 2) / (ti + tl + td) / 1e6, 2)}
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": round(keys_done / total / 1e6, 2),
         "unit": "Mkeys/s",
@@ -126,6 +154,8 @@ def cpu_baseline(n_ins, n_look, n_del, budget_s=25.0):
         "lookup_mkeys_s": round(nq / t_look / 1e6, 2),
         "delete_mkeys_s": round(nd / t_del / 1e6, 2),
         "seconds": round(total, 2),
+        "capacity_2p20": c1,
+        "host": {"nproc": os.cpu_count(), "model": model},
     }
 
 
@@ -352,7 +382,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sorted", action="store_true", help="also time insertManySorted")
+    ap.add_argument("--no-load-sweep", action="store_true",
+                    help="skip the insertMany / insertManySorted / lookup / delete sweep over load factors")
     ap.add_argument("--no-roofline", action="store_true", help="skip the measured gather/CAS roofline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-filter comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency runs")
@@ -433,23 +464,53 @@ def main():
     # reported beside it with the same byte accounting.
     roofline = _roofline("containsKernel", BYTES_PER_KEY["lookup"], N_LOOKUP, avg["lookup"])
     roofline_insert = _roofline("insertMany pipeline", BYTES_PER_KEY["insert"], n_ins, avg["insert"])
-    roofline_insert["kernels"] = ("packed-entry LSD partition (count + 2 scatter passes), binInsertKernel, "
-                                  "secondaryKernel, bfsStageKernel, dfsStageKernel")
+    roofline_insert["kernels"] = ("two-level partition (wide-tile count + scatter over hashed keys, then "
+                                  "per-partition count + scatter), binInsertKernel, secondaryKernel, "
+                                  "bfsStageKernel, dfsStageKernel")
 
     log(f"plain steps done: {ms_step:.2f} ms/step")
-    if args.sorted:
-        ev = _events(torch)
-        ts = []
-        for i in range(args.warmup + args.steps):
-            f.clear()
+    load_sweep = None
+    if not args.no_load_sweep:
+        # The same table filled from empty to each load factor, by insertMany and
+        # then by insertManySorted (each the mean of 3 timed calls after one
+        # untimed), then 2^27 lookups (half inserted keys, half never inserted)
+        # and a delete of a quarter of the inserted keys, on the filled table.
+        log("load sweep: alpha 0.5 / 0.8 / 0.9 / 0.95")
+        load_sweep = {}
+        negatives = keys[n_ins:]
+        for alpha in (0.5, 0.8, 0.9, 0.95):
+            n_a = int(CAPACITY * alpha)
+            ka = keys[:n_a]
+            row = {"keys": n_a}
+            for name, ins in (("insert", f.insertMany), ("insert_sorted", f.insertManySorted)):
+                ts = []
+                for i in range(4):
+                    f.clear()
+                    ev = _events(torch)
+                    ev[0].record()
+                    occ_a = ins(ka)
+                    ev[1].record()
+                    torch.cuda.synchronize()
+                    if i:
+                        ts.append(ev[0].elapsed_time(ev[1]))
+                row[f"{name}_mkeys_s"] = round(n_a / (sum(ts) / len(ts)) / 1e3, 1)
+                row[f"{name}_placed"] = occ_a
+            q = torch.cat([ka[: N_LOOKUP // 2], negatives])
+            ev = _events(torch)
             ev[0].record()
-            f.insertManySorted(pos)
+            f.containsMany(q, out=out)
             ev[1].record()
             torch.cuda.synchronize()
-            if i >= args.warmup:
-                ts.append(ev[0].elapsed_time(ev[1]))
-        per_op["insert_sorted_mkeys_s"] = n_ins / (sum(ts) / len(ts)) / 1e3
-        per_op["insert_sorted_ms"] = sum(ts) / len(ts)
+            row["lookup_mkeys_s"] = round(N_LOOKUP / ev[0].elapsed_time(ev[1]) / 1e3, 1)
+            row["lookup_positives_found"] = bool(out[: N_LOOKUP // 2].all().item())
+            nd_a = n_a // 4
+            ev[0].record()
+            f.deleteMany(ka[:nd_a], out=dout[:nd_a])
+            ev[1].record()
+            torch.cuda.synchronize()
+            row["delete_mkeys_s"] = round(nd_a / ev[0].elapsed_time(ev[1]) / 1e3, 1)
+            load_sweep[str(alpha)] = row
+        per_op["insert_sorted_mkeys_s"] = load_sweep["0.95"]["insert_sorted_mkeys_s"]
 
     result = {
         "metric": METRIC,
@@ -478,6 +539,8 @@ def main():
         "checks": {"occupied_after_insert": checks.get("occupied"), "expected": n_ins,
                    "occupied_after_delete": checks.get("remaining")},
     }
+    if load_sweep is not None:
+        result["load_sweep"] = load_sweep
     del keys, pos, query, dels, out, dout
     f.close()
     torch.cuda.empty_cache()
