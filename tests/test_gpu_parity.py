@@ -333,10 +333,11 @@ def test_partition_levels_insert_invariants(cf, oracle_mod, kw, cap, mode, parti
     f.close()
 
 
+@pytest.mark.parametrize("mode", ["insert", "sorted"])
 @pytest.mark.parametrize("n", [1, 7, 4095, 4097, 12287, 12289, 40961, 100003])
 @pytest.mark.parametrize("kw", [dict(eviction="bfs"), dict(eviction="bfs", bits_per_tag=8, word_bytes=4)],
                          ids=["t16", "t8"])
-def test_two_level_partition_small_and_ragged_batches(cf, oracle_mod, kw, n, monkeypatch):
+def test_two_level_partition_small_and_ragged_batches(cf, oracle_mod, kw, n, mode, monkeypatch):
     """Batches far smaller than the table through the two-level partition
     (forced binned path, 2^24 slots: 1024 slices or more): single keys,
     sizes around the 4096-value tile and the 12288-value first-pass tile, and
@@ -351,9 +352,10 @@ def test_two_level_partition_small_and_ragged_batches(cf, oracle_mod, kw, n, mon
     oc = _oracle_cfg(O, c)
     f = cf.CuckooFilter(1 << 24, config=c)
     nb = f.getNumBuckets()
+    insert = f.insertManySorted if mode == "sorted" else f.insertMany
     host = O.gen_keys(900 + n, n)
     keys = _dev_keys(cf, host)
-    occ = f.insertMany(keys)
+    occ = insert(keys)
     assert occ == n
     table = assert_table_count(f, occ, n)
     _check_invariants(O, oc, f.exportTable(), host, table, nb)
@@ -361,7 +363,7 @@ def test_two_level_partition_small_and_ragged_batches(cf, oracle_mod, kw, n, mon
     # a second batch on top (the partition's scratch is reused)
     host2 = O.gen_keys(1900 + n, n)
     keys2 = _dev_keys(cf, host2)
-    assert f.insertMany(keys2) == 2 * n
+    assert insert(keys2) == 2 * n
     assert f.containsMany(keys2).all().item() and f.containsMany(keys).all().item()
     out = torch.zeros(n, dtype=torch.bool, device=keys.device)
     assert f.deleteMany(keys, out=out) == n and out.all().item()
