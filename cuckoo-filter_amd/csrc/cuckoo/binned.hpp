@@ -493,14 +493,19 @@ struct BinnedInsert {
                align256((numSlices(numBuckets) + 1) * sizeof(uint32_t)) + align256(3 * sizeof(unsigned long long));
     }
 
-    /// Whether the slice index splits into two digits whose second-level
-    /// entries fit in 32 bits: then the partition is two-level (MSD), and its
-    /// values are (bucket bits below the first digit, fingerprint) after the
-    /// first pass and (bucket inside the slice, fingerprint) after the second.
+    /// Whether the slice index splits into two digits: then the partition is
+    /// two-level (MSD).  Where they fit in 32 bits its values are (bucket bits
+    /// below the first digit, fingerprint) after the first pass and (bucket
+    /// inside the slice, fingerprint) after the second; otherwise they stay
+    /// whole (bucket << 32 | fingerprint) entries.
     static constexpr int kTagBits = 8 * sizeof(typename Config::TagType);
-    static bool twoLevel(int l2s, int sliceBits) {
-        return sliceBits > radix::kDigitBits && sliceBits <= 2 * radix::kDigitBits &&
-               l2s + (sliceBits - radix::kDigitBits) + kTagBits <= 32 && !detail::forceOnePassFromEnv();
+    static bool twoLevel(int sliceBits) {
+        return sliceBits > radix::kDigitBits && sliceBits <= 2 * radix::kDigitBits && !detail::forceOnePassFromEnv();
+    }
+    /// Whether the two-level partition's values fit in 32 bits (else they stay
+    /// whole 64-bit entries).
+    static bool twoLevelPacked(int l2s, int sliceBits) {
+        return l2s + (sliceBits - radix::kDigitBits) + kTagBits <= 32;
     }
 
     template <bool SortedSemantics>
@@ -526,7 +531,22 @@ struct BinnedInsert {
 
             EntrySource<Config, SortedSemantics> src{keys + off, v.numBuckets};
             const size_t lds = binLdsBytes((size_t(1) << l2s) * kBucketBytes, sizeof(BinRetryOf<Config>));
-            if (twoLevel(l2s, sliceBits)) {
+            if (twoLevel(sliceBits) && !twoLevelPacked(l2s, sliceBits)) {
+                // whole 64-bit entries: a holds the first pass's output, b the
+                // second's; then the lists as below
+                const int d2 = sliceBits - radix::kDigitBits;
+                zeroCountersKernel<unsigned long long><<<1, 64, 0, s>>>(counters);
+                e = radix::partitionTwoLevel<uint64_t>(src, m, 32 + l2s + d2, radix::Identity<uint64_t>{}, 32 + l2s,
+                                                       uint32_t(1) << d2, radix::Identity<uint64_t>{}, a, b, scratch,
+                                                       start, static_cast<uint32_t>(slices), s);
+                if (e != hipSuccess) return e;
+                binInsertKernel<Config, uint64_t><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(
+                    b, start, l2s, v, a, counters);
+                e = runStages(v, m, block, s, counters, a, b);
+                if (e != hipSuccess) return e;
+                continue;
+            }
+            if (twoLevel(sliceBits)) {
                 // a holds the first pass's 32-bit values, b the second's; after
                 // the primary attempts the overflow list goes to a, eviction
                 // starts to b, the DFS list to a

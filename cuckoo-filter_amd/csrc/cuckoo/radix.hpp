@@ -394,11 +394,13 @@ __global__ __launch_bounds__(kBlock, 6) void scatterKernel(Src src, typename P::
 template <int PARTS>
 inline constexpr int kWideTileOf = PARTS * kTile;
 
-/// Parts per wide tile in the two-level partition's first pass: three
-/// (48-value runs, 60 KiB of LDS, two workgroups per CU) measured faster than
-/// two (32-value runs, three workgroups per CU): 5.53 against 5.63 ms per
-/// 255M-key insert.  Four would leave one workgroup per CU.
-inline constexpr int kWideParts = 3;
+/// Parts per wide tile in the two-level partition's first pass, by output
+/// value size.  32-bit values: three (48-value runs, 60 KiB of LDS, two
+/// workgroups per CU) measured faster than two (32-value runs, three
+/// workgroups per CU): 5.53 against 5.63 ms per 255M-key insert; four would
+/// leave one workgroup per CU.  64-bit values: two (72 KiB, two per CU).
+template <typename O>
+inline constexpr int kWidePartsFor = sizeof(O) <= 4 ? 3 : 2;
 
 /// Digit histogram per wide tile (tile-major rows, as countKernel).
 template <typename U, typename Src, int PARTS>
@@ -688,7 +690,9 @@ __device__ __forceinline__ bool segmentTile(const uint32_t* __restrict__ partSta
 }
 
 /// Digit histogram of each segmented tile, one row of the tile-major matrix
-/// per tile (only the first `radix` columns are written).
+/// per tile (only the first `radix` columns are written).  The digit is the
+/// `radix` (a power of two) values at `shift`: bits above it, such as the
+/// first-level digit of whole entries, are masked off.
 template <typename U>
 __global__ __launch_bounds__(kBlock) void countSegKernel(const U* __restrict__ in, const uint32_t* __restrict__ partStart,
                                                          const uint32_t* __restrict__ tileBase, int shift, uint32_t radix,
@@ -708,7 +712,7 @@ __global__ __launch_bounds__(kBlock) void countSegKernel(const U* __restrict__ i
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint32_t idx = lo + static_cast<uint32_t>(k) * kBlock + threadIdx.x;
-        if (idx < hi) atomicAdd(&hist[static_cast<uint32_t>(raw[k] >> shift) & (kRadix - 1)], 1u);
+        if (idx < hi) atomicAdd(&hist[static_cast<uint32_t>(raw[k] >> shift) & (radix - 1)], 1u);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < radix; i += kBlock) counts[static_cast<size_t>(blockIdx.x) * kRadix + i] = hist[i];
@@ -747,12 +751,14 @@ __global__ __launch_bounds__(kRadix) void segScanKernel(C* __restrict__ counts, 
     if (p == 0 && d == 0) groupStart[numGroups] = partStart[kRadix];
 }
 
-/// The segmented pass's scatter: one segmented tile per workgroup.
+/// The segmented pass's scatter: one segmented tile per workgroup; digits as
+/// in countSegKernel.
 template <typename U, typename P>
 __global__ __launch_bounds__(kBlock) void scatterSegKernel(const U* __restrict__ in, typename P::Out* __restrict__ out,
                                                            const uint32_t* __restrict__ partStart,
                                                            const uint32_t* __restrict__ tileBase, int shift,
-                                                           const uint32_t* __restrict__ offsets, P pack) {
+                                                           uint32_t radix, const uint32_t* __restrict__ offsets,
+                                                           P pack) {
     using O = typename P::Out;
     extern __shared__ __align__(16) unsigned char ldsRaw[];
     O* stage = reinterpret_cast<O*>(ldsRaw);
@@ -772,7 +778,7 @@ __global__ __launch_bounds__(kBlock) void scatterSegKernel(const U* __restrict__
     for (int r = 0; r < kRowsPerWave; ++r) {
         const uint32_t idx = waveBase + static_cast<uint32_t>(r) * 64 + lane;
         const U v = in[idx < last ? idx : last];  // past-the-end lanes are masked by tileCount
-        digits |= static_cast<uint64_t>(static_cast<uint32_t>(v >> shift) & (kRadix - 1)) << (r * kDigitBits);
+        digits |= static_cast<uint64_t>(static_cast<uint32_t>(v >> shift) & (radix - 1)) << (r * kDigitBits);
         vals[r] = pack(v);
     }
     const uint32_t myBase =
@@ -852,8 +858,9 @@ inline size_t twoLevelScratchBytes(size_t n) {
 ///     at `shift1` into partitions p and written to `a` through `pack1`;
 ///  2. each partition is split by the digit at `shift2` of the packed value
 ///     (radix2 <= kRadix values) and written to `b` through `pack2`.
-/// groupStart[g] receives the start of group g for g < numGroups, and
-/// groupStart[numGroups] = n.  n must be below 2^31.
+/// The packed values may be 32 or 64 bits wide.  groupStart[g] receives the
+/// start of group g for g < numGroups, and groupStart[numGroups] = n.  n must
+/// be below 2^31.
 template <typename U, typename Src, typename P1, typename P2>
 inline hipError_t partitionTwoLevel(Src first, size_t n, int shift1, P1 pack1, int shift2, uint32_t radix2, P2 pack2,
                                     typename P1::Out* a, typename P2::Out* b, void* scratch, uint32_t* groupStart,
@@ -868,7 +875,7 @@ inline hipError_t partitionTwoLevel(Src first, size_t n, int shift1, P1 pack1, i
     uint32_t* partial = chunkSum + static_cast<size_t>(chunks) * kRadix;
     uint32_t* partStart = partial + (static_cast<size_t>(chunks) * kRadix + kScanChunk - 1) / kScanChunk;
     uint32_t* tileBase = partStart + kRadix + 1;
-    constexpr int PARTS = kWideParts;
+    constexpr int PARTS = kWidePartsFor<V>;
     const uint32_t wideTiles = static_cast<uint32_t>((n + kWideTileOf<PARTS> - 1) / kWideTileOf<PARTS>);
     countWideKernel<U, Src, PARTS><<<wideTiles, kBlock, 0, stream>>>(first, n, shift1, counts);
     scanCounts(counts, wideTiles, chunkSum, partial, stream);
@@ -878,9 +885,8 @@ inline hipError_t partitionTwoLevel(Src first, size_t n, int shift1, P1 pack1, i
     segmentKernel<uint32_t><<<1, kRadix, 0, stream>>>(counts, static_cast<uint32_t>(n), partStart, tileBase);
     countSegKernel<V><<<tiles2, kBlock, 0, stream>>>(a, partStart, tileBase, shift2, radix2, counts);
     segScanKernel<uint32_t><<<kRadix, kRadix, 0, stream>>>(counts, partStart, tileBase, radix2, groupStart, numGroups);
-    scatterSegKernel<V, P2><<<tiles2, kBlock, unstableLdsBytes<typename P2::Out>(), stream>>>(a, b, partStart,
-                                                                                               tileBase, shift2,
-                                                                               counts, pack2);
+    scatterSegKernel<V, P2><<<tiles2, kBlock, unstableLdsBytes<typename P2::Out>(), stream>>>(
+        a, b, partStart, tileBase, shift2, radix2, counts, pack2);
     return hipGetLastError();
 }
 
