@@ -82,8 +82,11 @@ def _roofline(kernel, bytes_per_key, n, ms):
     }
 
 
+_T0 = time.perf_counter()
+
+
 def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def _events(torch):
