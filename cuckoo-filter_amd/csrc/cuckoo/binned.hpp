@@ -452,7 +452,11 @@ inline InsertPath insertPathFromEnv() {
     return InsertPath::Auto;
 }
 
-inline constexpr size_t kBinnedMinBucketsPerKey = 4;
+/// The binned path reads and writes every table slice it has entries for, so
+/// it pays off once a batch is large against the table: measured at 2^28
+/// slots, the two paths cost the same at about one key per two buckets for
+/// fills of 0.3 to 0.8 (the binned path wins from about one per three at 0.9).
+inline constexpr size_t kBinnedMinBucketsPerKey = 2;
 
 /// CUCKOO_PARTITION=lsd keeps the one-level-per-pass LSD partition even where
 /// the two-level one applies (for comparisons).
