@@ -1,4 +1,6 @@
-// LSD radix partition over a bit range of 32/64-bit values, written for wave64.
+// Radix partitions over a bit range of 32/64-bit values, written for wave64:
+// LSD passes of one digit each (sortBits) and a two-level MSD partition
+// (partitionTwoLevel).
 //
 // Used by the bucket-binned insert (binned.hpp) to group packed
 // (primary bucket, fingerprint) entries by the table slice their primary
@@ -28,8 +30,9 @@
 // each partition is split by the second digit on its own, so the bits of
 // the first digit can be dropped from the values (a scatter may write its
 // values through an output transform), and the group starts come out of
-// the second pass's scan.  Its first pass works on wide tiles (kWideParts
-// tiles of kTile values) for longer output runs per digit.
+// the second pass's scan.  Neither of its passes needs a stable order.  Its
+// first pass works on wide tiles (kWidePartsFor<value> tiles of kTile
+// values) for longer output runs per digit.
 #pragma once
 
 #include <hip/hip_runtime.h>
