@@ -285,6 +285,10 @@ TWO_LEVEL_CFGS = [
     (dict(eviction="bfs", bits_per_tag=8, bucket_size=4, word_bytes=4), 1 << 24),   # 4-byte buckets, 2^13-bucket slices
     (dict(eviction="bfs", bits_per_tag=32, bucket_size=8), 1 << 23),      # too wide: LSD
     (dict(eviction="bfs", policy="addsub"), 1 << 23),
+    # non-power-of-two bucket counts: 781,250 buckets in 763 slices (the last
+    # partial), 393,988 in 385; some first-level partitions stay empty
+    (dict(eviction="bfs", policy="addsub"), 12_500_000),
+    (dict(eviction="dfs", policy="offset"), 6_303_801),
 ]
 
 
