@@ -76,6 +76,9 @@ def _roofline(kernel, bytes_per_key, n, ms):
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
+        # physical HBM bytes per second: the committed per-launch traffic over
+        # this run's launch time (a random 32-byte read moves a 128-byte line)
+        "traffic_gbs": round(traffic / (ms / 1e3) / 1e9, 1) if traffic else None,
         "algorithmic_bytes": int(bytes_per_key * n),
         "bytes_per_key": round(bytes_per_key, 2),
         "ms_per_launch": round(ms, 3),
