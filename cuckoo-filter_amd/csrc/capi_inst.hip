@@ -93,9 +93,9 @@ struct Entry {
         return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->execute(ops, n, s, occupiedOut);
     }
     static hipError_t batchLaunch(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t s,
-                                  bool useGraph) {
+                                  bool useGraph, uint32_t chunkKeys) {
         if (*state == nullptr) *state = new cuckoo::BatchExecutor<Cfg>();
-        return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->launch(ops, n, s, useGraph);
+        return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->launch(ops, n, s, useGraph, chunkKeys);
     }
     static const unsigned long long* batchResults(const void* state) {
         return static_cast<const cuckoo::BatchExecutor<Cfg>*>(state)->results();

@@ -43,7 +43,8 @@ struct Ops {
                                uint64_t* occupiedOut);
     // asynchronous form: launches on the stream; batchResults gives the
     // post-op occupancies once the stream has passed the launch
-    hipError_t (*batchLaunch)(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t, bool useGraph);
+    hipError_t (*batchLaunch)(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t, bool useGraph,
+                              uint32_t chunkKeys);
     const unsigned long long* (*batchResults)(const void* state);
     uint32_t (*batchLevels)(const void* state);
     void (*batchRelease)(void* state);

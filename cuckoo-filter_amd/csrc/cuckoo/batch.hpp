@@ -9,9 +9,12 @@
 // a *level*: a lookup runs one level after the last earlier insert/delete on
 // its filter, an insert/delete one level after every earlier batch on its
 // filter.  The levels run in order, each as one grid-stride kernel over
-// 512-key chunks of all of its batches' keys.  The last workgroup to finish a
-// level records every batch's filter occupancy, which is exactly the value
-// the plain insertMany / deleteMany call would have returned.
+// fixed-size key chunks of all of its batches' keys (512 keys, two per thread,
+// for throughput; the serving loop uses 256, one per thread, for latency).
+// In a level with an insert or delete, the last workgroup to finish records
+// every batch's filter occupancy, which is exactly the value the plain
+// insertMany / deleteMany call would have returned; a level of lookups only
+// changes no counter, so its first workgroup records them at once.
 //
 // A window is one device buffer: a header, the batch descriptors, the
 // level tables and per-level completion counters, uploaded with one copy
@@ -55,7 +58,8 @@ struct BatchOpDesc {
     uint32_t pad;
 };
 
-inline constexpr uint32_t kBatchChunk = 512;  // keys per workgroup step: two per thread
+inline constexpr uint32_t kBatchChunk = 512;        // default keys per workgroup step: two per thread
+inline constexpr uint32_t kBatchChunkLatency = 256;  // one key per thread (serving windows)
 inline constexpr int kBatchBlock = 256;
 inline constexpr uint32_t kMaxGraphLevels = 64;  // deeper windows launch level by level
 
@@ -66,9 +70,10 @@ struct alignas(64) WindowHeader {
     uint32_t pad[2];
     uint64_t opsOff;       // BatchOpDesc[numOps]
     uint64_t levelOpsOff;  // uint32_t[numOps]: op indices grouped by level, window order inside a level
-    uint64_t chunkCumOff;  // uint32_t[numOps + 1]: kBatchChunk-key chunks before each levelOps position
+    uint64_t chunkCumOff;  // uint32_t[numOps + 1]: chunks before each levelOps position
     uint64_t opStartOff;   // uint32_t[numLevels + 1]: first levelOps position of each level
     uint64_t doneOff;      // uint32_t[numLevels]: workgroups finished per level (uploaded as zero)
+    uint64_t levelMutOff;  // uint32_t[numLevels]: 1 if the level holds an insert or delete
 };
 
 namespace detail {
@@ -81,8 +86,10 @@ __device__ __forceinline__ T* windowPart(unsigned char* w, uint64_t off) {
 /// One level: a grid-stride loop over the level's chunks (each workgroup
 /// finds its batch by binary search over the chunk counts), then the last
 /// workgroup to finish records results[op] = the op's filter occupancy for
-/// every op of the level.
-template <typename Config>
+/// every op of the level.  A level of lookups only leaves every counter as
+/// the previous level (an earlier kernel on the stream) left it, so
+/// workgroup 0 records them first and no completion count is kept.
+template <typename Config, uint32_t Chunk>
 __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* __restrict__ window, uint32_t level,
                                                                 unsigned long long* __restrict__ results) {
     using Ops = FilterOps<Config>;
@@ -93,9 +100,16 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* _
     const auto* chunkCum = detail::windowPart<const uint32_t>(window, hdr->chunkCumOff);
     const auto* opStart = detail::windowPart<const uint32_t>(window, hdr->opStartOff);
     auto* done = detail::windowPart<uint32_t>(window, hdr->doneOff);
+    const bool mutates = detail::windowPart<const uint32_t>(window, hdr->levelMutOff)[level] != 0;
     const uint32_t p0 = opStart[level];
     const uint32_t p1 = opStart[level + 1];
     const uint32_t g1 = chunkCum[p1];
+    if (!mutates && blockIdx.x == 0) {
+        for (uint32_t j = p0 + threadIdx.x; j < p1; j += kBatchBlock) {
+            const uint32_t op = levelOps[j];
+            results[op] = __hip_atomic_load(ops[op].occupied, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
     for (uint32_t g = chunkCum[p0] + blockIdx.x; g < g1; g += gridDim.x) {
         // last position j in [p0, p1) with chunkCum[j] <= g (skips empty batches)
         uint32_t lo = p0;
@@ -109,8 +123,8 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* _
             }
         }
         const BatchOpDesc& o = ops[levelOps[lo]];
-        const uint64_t offset = static_cast<uint64_t>(g - chunkCum[lo]) * kBatchChunk;
-        const uint32_t count = static_cast<uint32_t>(o.n - offset < kBatchChunk ? o.n - offset : kBatchChunk);
+        const uint64_t offset = static_cast<uint64_t>(g - chunkCum[lo]) * Chunk;
+        const uint32_t count = static_cast<uint32_t>(o.n - offset < Chunk ? o.n - offset : Chunk);
         DeviceView<Config> v;
         v.words = static_cast<typename DeviceView<Config>::Word*>(o.words);
         v.numBuckets = o.numBuckets;
@@ -153,6 +167,7 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* _
     // workgroup's counter updates precede its increment of done[level].  Only
     // workgroups that own a chunk (block b < the level's chunk count) take
     // part; with no chunk at all, workgroup 0 snapshots alone.
+    if (!mutates) return;
     const uint32_t levelChunks = g1 - chunkCum[p0];
     const uint32_t parts = levelChunks < gridDim.x ? levelChunks : gridDim.x;
     if (parts == 0 ? blockIdx.x != 0 : blockIdx.x >= parts) return;
@@ -206,9 +221,14 @@ class BatchExecutor {
     /// cached for this level count (cheapest when windows of one shape
     /// repeat); otherwise level by level, each with a grid no larger than its
     /// chunk count (cheapest when every window differs, as in a server).
-    hipError_t launch(const BatchOpDesc* ops, size_t count, hipStream_t stream, bool useGraph = true) {
+    /// chunkKeys is the keys per workgroup step: kBatchChunk (two per
+    /// thread) or kBatchChunkLatency (one per thread); graphs are kept for
+    /// kBatchChunk only, other windows run level by level.
+    hipError_t launch(const BatchOpDesc* ops, size_t count, hipStream_t stream, bool useGraph = true,
+                      uint32_t chunkKeys = kBatchChunk) {
         if (count == 0) return hipSuccess;
-        const uint32_t numLevels = plan(ops, count);
+        if (chunkKeys != kBatchChunk && chunkKeys != kBatchChunkLatency) return hipErrorInvalidValue;
+        const uint32_t numLevels = plan(ops, count, chunkKeys);
         if (stageError_ != hipSuccess) return stageError_;
         hipError_t e;
         bool moved = false;
@@ -220,7 +240,7 @@ class BatchExecutor {
         if ((e = hipMemcpyAsync(dWin_, hStage_, staged_, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
         auto* win = static_cast<unsigned char*>(dWin_);
         auto* results = static_cast<unsigned long long*>(dResults_);
-        if (useGraph && numLevels <= kMaxGraphLevels) {
+        if (useGraph && chunkKeys == kBatchChunk && numLevels <= kMaxGraphLevels) {
             if (graphs_[numLevels] == nullptr && (e = capture(numLevels)) != hipSuccess) return e;
             if ((e = hipGraphLaunch(graphs_[numLevels], stream)) != hipSuccess) return e;
         } else {
@@ -229,7 +249,11 @@ class BatchExecutor {
             for (uint32_t l = 0; l < numLevels; ++l) {
                 const uint32_t chunks = chunkCum_[opStart_[l + 1]] - chunkCum_[opStart_[l]];
                 const uint32_t g = chunks == 0 ? 1u : (chunks < grid ? chunks : grid);
-                batchLevelKernel<Config><<<g, kBatchBlock, 0, stream>>>(win, l, results);
+                if (chunkKeys == kBatchChunk) {
+                    batchLevelKernel<Config, kBatchChunk><<<g, kBatchBlock, 0, stream>>>(win, l, results);
+                } else {
+                    batchLevelKernel<Config, kBatchChunkLatency><<<g, kBatchBlock, 0, stream>>>(win, l, results);
+                }
             }
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
@@ -261,7 +285,7 @@ class BatchExecutor {
 
     /// Writes the window (header, descriptors, chunks, level tables) into the
     /// pinned staging buffer; returns the level count.
-    uint32_t plan(const BatchOpDesc* ops, size_t count) {
+    uint32_t plan(const BatchOpDesc* ops, size_t count, uint32_t chunk) {
         seen_.clear();
         level_.resize(count);
         uint32_t numLevels = 0;
@@ -295,9 +319,19 @@ class BatchExecutor {
         // chunk counts in level order
         chunkCum_.resize(count + 1);
         chunkCum_[0] = 0;
+        uint64_t chunks = 0;
         for (size_t j = 0; j < count; ++j) {
-            const uint64_t n = ops[levelOps_[j]].n;
-            chunkCum_[j + 1] = chunkCum_[j] + static_cast<uint32_t>((n + kBatchChunk - 1) / kBatchChunk);
+            chunks += (ops[levelOps_[j]].n + chunk - 1) / chunk;
+            if (chunks > UINT32_MAX) {
+                stageError_ = hipErrorInvalidValue;  // more chunks than the 32-bit tables index
+                staged_ = 0;
+                return 0;
+            }
+            chunkCum_[j + 1] = static_cast<uint32_t>(chunks);
+        }
+        levelMut_.assign(numLevels, 0);
+        for (size_t i = 0; i < count; ++i) {
+            if (ops[i].kind != static_cast<uint32_t>(BatchOpKind::Lookup)) levelMut_[level_[i]] = 1;
         }
 
         WindowHeader h{};
@@ -314,6 +348,8 @@ class BatchExecutor {
         off = align64(off + (numLevels + 1) * sizeof(uint32_t));
         h.doneOff = off;
         off = align64(off + numLevels * sizeof(uint32_t));
+        h.levelMutOff = off;
+        off = align64(off + numLevels * sizeof(uint32_t));
         staged_ = off;
         stageError_ = growPinned(&hStage_, &capStage_, staged_);
         if (stageError_ != hipSuccess) {
@@ -327,6 +363,7 @@ class BatchExecutor {
         std::memcpy(b + h.chunkCumOff, chunkCum_.data(), (count + 1) * sizeof(uint32_t));
         std::memcpy(b + h.opStartOff, opStart_.data(), (numLevels + 1) * sizeof(uint32_t));
         std::memset(b + h.doneOff, 0, numLevels * sizeof(uint32_t));
+        std::memcpy(b + h.levelMutOff, levelMut_.data(), numLevels * sizeof(uint32_t));
         return numLevels;
     }
 
@@ -364,7 +401,7 @@ class BatchExecutor {
     }
 
     static uint32_t levelGrid() {
-        return detail::gridFor(batchLevelKernel<Config>, static_cast<size_t>(1) << 30, kBatchBlock);
+        return detail::gridFor(batchLevelKernel<Config, kBatchChunk>, static_cast<size_t>(1) << 30, kBatchBlock);
     }
 
     void dropGraphs() {
@@ -383,7 +420,7 @@ class BatchExecutor {
         const uint32_t grid = levelGrid();
         if ((e = hipStreamBeginCapture(captureStream_, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
         for (uint32_t l = 0; l < numLevels; ++l) {
-            batchLevelKernel<Config><<<grid, kBatchBlock, 0, captureStream_>>>(win, l, results);
+            batchLevelKernel<Config, kBatchChunk><<<grid, kBatchBlock, 0, captureStream_>>>(win, l, results);
         }
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(captureStream_, &graph);
@@ -411,7 +448,7 @@ class BatchExecutor {
     // planning scratch (host)
     std::unordered_map<const void*, uint32_t> seen_;
     std::vector<FilterLevelsPod> state_;
-    std::vector<uint32_t> level_, opStart_, levelOps_, chunkCum_, cursor_;
+    std::vector<uint32_t> level_, opStart_, levelOps_, chunkCum_, cursor_, levelMut_;
     // graphs by level count
     hipStream_t captureStream_ = nullptr;
     hipGraphExec_t graphs_[kMaxGraphLevels + 1] = {};

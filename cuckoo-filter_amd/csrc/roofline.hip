@@ -13,6 +13,12 @@
 //    (hardware register XCC_ID) and only touches that XCD's eighth of the
 //    table, so no line is shared between XCDs; with workgroup scope the CAS
 //    can then complete in the XCD's own L2 (agent scope for comparison).
+//  * rl_gather_region: per key, one random 32-byte read confined to the table
+//    region its position in the key array falls in (keys grouped by region,
+//    as a lookup whose keys were partitioned by bucket range would see them).
+//    Routed: each XCD takes the contiguous eighth of the regions it owns
+//    (XCC_ID) from its own work counter, so a region's lines stay in that
+//    XCD's L2; otherwise all workgroups take positions in order.
 //  * rl_store_bytes: per key, one byte stored at a random position (the
 //    output of a lookup or delete whose keys were reordered).
 //  * rl_copy: a plain streaming copy, used to calibrate the FETCH_SIZE /
@@ -86,6 +92,43 @@ __global__ __launch_bounds__(1024) void casXcdKernel(unsigned long long* __restr
         unsigned long long* p = base + (mix(k) & regionMask);
         unsigned long long expected = 0;
         __hip_atomic_compare_exchange_strong(p, &expected, k | 1ull, __ATOMIC_RELAXED, __ATOMIC_RELAXED, SCOPE);
+    }
+}
+
+constexpr uint32_t kRegionChunk = 1024;
+__device__ uint32_t g_regionNext[8];  // per-XCD work counters of regionGatherKernel
+
+__global__ void zeroRegionCountersKernel() {
+    if (threadIdx.x < 8) g_regionNext[threadIdx.x] = 0;
+}
+
+// Workgroups take kRegionChunk positions at a time from their queue's counter;
+// routed, a workgroup starts at its own XCD's queue and then helps the others,
+// so every position is read whatever the placement of workgroups on XCDs.
+__global__ __launch_bounds__(256) void regionGatherKernel(const V4* __restrict__ table, uint64_t regionBlocks,
+                                                          uint64_t regions, const uint64_t* __restrict__ keys,
+                                                          size_t n, uint8_t* __restrict__ out, int routed) {
+    __shared__ uint32_t sBase;
+    const unsigned queues = routed ? 8u : 1u;
+    const unsigned home = routed ? (xccId() & 7u) : 0u;
+    for (unsigned q = 0; q < queues; ++q) {
+        const unsigned x = (home + q) % queues;
+        const size_t lo = n * x / queues;
+        const size_t hi = n * (x + 1) / queues;
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) sBase = atomicAdd(&g_regionNext[x], kRegionChunk);
+            __syncthreads();
+            const size_t b = sBase;
+            if (b >= hi - lo) break;
+            const size_t end = hi - lo - b < kRegionChunk ? hi - lo - b : kRegionChunk;
+            for (size_t j = threadIdx.x; j < end; j += blockDim.x) {
+                const size_t i = lo + b + j;
+                const uint64_t r = static_cast<uint64_t>(i) * regions / n;
+                const uint64_t blk = r * regionBlocks + (mix(keys[i]) & (regionBlocks - 1));
+                out[i] = readBlock<32, false>(table + blk * 2) == 0x5A5A5A5Au;
+            }
+        }
     }
 }
 
@@ -172,6 +215,20 @@ int rl_cas_xcd(void* table, size_t table_bytes, unsigned regions, int workgroup_
         auto k = casXcdKernel<__HIP_MEMORY_SCOPE_AGENT>;
         k<<<gridFor((const void*)k, n, block), block, 0, s>>>(w, perRegion - 1, shift, regions, keys, n);
     }
+    return static_cast<int>(hipGetLastError());
+}
+
+// region_bytes: a power of two, at least 32 bytes, with table_bytes a multiple
+// of 8 regions; n below 2^31.
+int rl_gather_region(const void* table, size_t table_bytes, size_t region_bytes, const uint64_t* keys, size_t n,
+                     uint8_t* out, int routed, int grid, void* stream) {
+    if (!table || !keys || !out || n == 0 || n >= (size_t(1) << 31) || grid < 1 || !pow2(region_bytes) ||
+        region_bytes < 32 || table_bytes % region_bytes || (table_bytes / region_bytes) % 8)
+        return -1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    zeroRegionCountersKernel<<<1, 64, 0, s>>>();
+    regionGatherKernel<<<grid, 256, 0, s>>>(static_cast<const V4*>(table), region_bytes / 32,
+                                            table_bytes / region_bytes, keys, n, out, routed);
     return static_cast<int>(hipGetLastError());
 }
 

@@ -36,6 +36,7 @@ constexpr size_t kStageAlign = 256;
 
 size_t alignUp(size_t x) { return (x + kStageAlign - 1) & ~(kStageAlign - 1); }
 
+
 uint64_t nowNs() {
     return static_cast<uint64_t>(
         std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
@@ -148,8 +149,13 @@ int cf_server::launchWindow(Slot& sl) {
     hipError_t e = hipSuccess;
     if (keyBytes != 0) e = hipMemcpyAsync(d, h, keyBytes, hipMemcpyHostToDevice, stream);
     void** st = static_cast<void**>(executorFor(sl, sl.ops));
-    // windows differ from one to the next: level by level, each level's grid sized to its work
-    if (e == hipSuccess) e = sl.ops->batchLaunch(st, sl.descs.data(), sl.descs.size(), stream, false);
+    // Windows differ from one to the next: level by level, each level's grid
+    // sized to its work, and one key per thread, so that a small request's
+    // keys run side by side (a level lasts as long as its slowest workgroup).
+    if (e == hipSuccess) {
+        e = sl.ops->batchLaunch(st, sl.descs.data(), sl.descs.size(), stream, false,
+                                cuckoo::kBatchChunkLatency);
+    }
     sl.state = *st;
     if (e == hipSuccess && outBytes > keyBytes)
         e = hipMemcpyAsync(h + keyBytes, d + keyBytes, outBytes - keyBytes, hipMemcpyDeviceToHost, stream);

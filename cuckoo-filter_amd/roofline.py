@@ -10,6 +10,9 @@ Runs the kernels of libcuckoo_roofline.so with HIP events:
   cas8                : one random 64-bit CAS per key (device scope);
   cas8 xcd            : the same, each workgroup confined to the eighth of the
                         table its XCD owns, at workgroup and at device scope;
+  gather32 region     : gather32 with the keys grouped by 1 MiB table region,
+                        routed to the XCD that owns the region and in order
+                        (reported as G requests/s beside random gather32);
   store1 random       : one byte stored at a random place of a 64 MiB array;
   copy                : streaming copy (HBM bandwidth reference).
 Each figure is algorithmic bytes (the bytes named above) per second.
@@ -40,7 +43,8 @@ def _lib():
     L.rl_cas_xcd.argtypes = [P, sz, C.c_uint, i, P, sz, i, P]
     L.rl_store_bytes.argtypes = [P, sz, P, sz, i, P]
     L.rl_copy.argtypes = [P, P, sz, i, P]
-    for f in (L.rl_gather, L.rl_cas, L.rl_cas_xcd, L.rl_store_bytes, L.rl_copy):
+    L.rl_gather_region.argtypes = [P, sz, sz, P, sz, P, i, i, P]
+    for f in (L.rl_gather, L.rl_cas, L.rl_cas_xcd, L.rl_store_bytes, L.rl_copy, L.rl_gather_region):
         f.restype = i
     return L
 
@@ -87,6 +91,17 @@ def measure(table_bytes=512 << 20, n=1 << 26, reps=5, block=256, big_factor=4) -
         res[f"gather128_{tag}_gbs"] = round(gather(tab, tb, 128, 1, 0), 1)
         res[f"gather32nt_{tag}_gbs"] = round(gather(tab, tb, 32, 1, 0, 1), 1)
         res[f"gather32x2dep_{tag}_gbs"] = round(gather(tab, tb, 32, 2, 1), 1)
+        if factor == 1:
+            res[f"gather32_{tag}_greq"] = round(res[f"gather32_{tag}_gbs"] / (8 + 1 + 32), 1)
+            props = torch.cuda.get_device_properties(dev)
+            grid = 8 * props.multi_processor_count
+            for routed in (1, 0):
+                def region():
+                    rc = L.rl_gather_region(C.c_void_p(tab.data_ptr()), tb, 1 << 20, C.c_void_p(keys.data_ptr()),
+                                            n, C.c_void_p(out.data_ptr()), routed, grid, s)
+                    assert rc == 0, rc
+                t = _time(torch, region, reps)
+                res[f"gather32_region1MiB_{'xcd' if routed else 'inorder'}_{tag}_greq"] = round(n / t / 1e9, 1)
 
         def cas():
             rc = L.rl_cas(C.c_void_p(tab.data_ptr()), tb, C.c_void_p(keys.data_ptr()), n, block, s)
