@@ -143,6 +143,28 @@ def cpu_baseline(n_ins, n_look, n_del, budget_s=25.0):
         c1[label] = {"insert_mkeys_s": round(n1 / ti / 1e6, 2), "lookup_mkeys_s": round(q1.size / tl / 1e6, 2),
                      "delete_mkeys_s": round((n1 // 2) / td / 1e6, 2),
                      "mkeys_s": round((n1 + q1.size + n1 // 2) / (ti + tl + td) / 1e6, 2)}
+    # a restatement of the CPU filter the reference compares against
+    # (cuckoofilter::CuckooFilter<uint64_t, 16>, benchmark/cuckoo_cpu_vs_gpu.cu:
+    # 52-155), on that harness's workload: insert n = 95% of the capacity,
+    # look the same n keys up, delete them all; one thread
+    cc = {}
+    for label, cap in (("capacity_2p20", 1 << 20), ("capacity_2p24", 1 << 24)):
+        g = oracle.CpuCuckoo(cap)
+        n1 = int(cap * 0.95)
+        k1 = oracle.gen_keys(SEED + 2, n1)
+        t0 = time.perf_counter()
+        added = g.add_many(k1)
+        ti = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        found = g.contain_many(k1)
+        tl = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        gone = g.delete_many(k1)
+        td = time.perf_counter() - t0
+        cc[label] = {"keys": n1, "table_bytes": g.size_bytes(), "added": added, "found": found, "deleted": gone,
+                     "insert_mkeys_s": round(n1 / ti / 1e6, 2), "lookup_mkeys_s": round(n1 / tl / 1e6, 2),
+                     "delete_mkeys_s": round(n1 / td / 1e6, 2), "mkeys_s": round(3 * n1 / (ti + tl + td) / 1e6, 2)}
+        del g
     model = ""
     try:
         with open("/proc/cpuinfo") as fh:
@@ -161,6 +183,11 @@ def cpu_baseline(n_ins, n_look, n_del, budget_s=25.0):
         "delete_mkeys_s": round(nd / t_del / 1e6, 2),
         "seconds": round(total, 2),
         "capacity_2p20": c1,
+        "cpu_cuckoo_restatement": {
+            "what": ("single-threaded restatement of the published cuckoo filter design behind "
+                     "cuckoofilter::CuckooFilter<uint64_t,16> (4 x 16-bit slots per bucket, victim cache), "
+                     "not upstream code; insert 95% of capacity, look them up, delete them"),
+            **cc},
         "host": {"nproc": os.cpu_count(), "model": model},
     }
 

@@ -89,6 +89,19 @@ def lib():
             "cfo_reset_stats": (None, [P]),
             "cfo_swar_zero_mask": (u64, [u64, u32, u32]),
             "cfo_swar_replicate": (u64, [u32, u32, u32]),
+            # the single-threaded CPU filter of cpu_cuckoo.h (baseline only)
+            "ccf_create": (P, [u64]),
+            "ccf_destroy": (None, [P]),
+            "ccf_num_buckets": (u64, [P]),
+            "ccf_size_bytes": (u64, [P]),
+            "ccf_items": (u64, [P]),
+            "ccf_victim_used": (C.c_int, [P]),
+            "ccf_alt_index": (u64, [P, u64, u32]),
+            "ccf_index_tag": (None, [P, u64, C.POINTER(u64), C.POINTER(u32)]),
+            "ccf_checksum": (u64, [P]),
+            "ccf_add_many": (u64, [P, P, sz, P]),
+            "ccf_contain_many": (u64, [P, P, sz, P]),
+            "ccf_delete_many": (u64, [P, P, sz, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -242,6 +255,62 @@ class Filter:
         assert img.size == self.size_bytes
         C.memmove(lib().cfo_table(self._h), img.ctypes.data, img.size)
         lib().cfo_set_occupied(self._h, occupied)
+
+
+class CpuCuckoo:
+    """The single-threaded CPU cuckoo filter of cpu_cuckoo.h: a restatement of
+    the published design behind cuckoofilter::CuckooFilter<uint64_t, 16>, the
+    CPU filter the reference compares against
+    (benchmark/cuckoo_cpu_vs_gpu.cu:52-155).  A reported baseline only."""
+
+    def __init__(self, max_keys: int):
+        self._h = lib().ccf_create(max_keys)
+        if not self._h:
+            raise MemoryError("ccf_create failed")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().ccf_destroy(h)
+            self._h = None
+
+    def num_buckets(self) -> int:
+        return int(lib().ccf_num_buckets(self._h))
+
+    def size_bytes(self) -> int:
+        return int(lib().ccf_size_bytes(self._h))
+
+    def items(self) -> int:
+        return int(lib().ccf_items(self._h))
+
+    def victim_used(self) -> bool:
+        return bool(lib().ccf_victim_used(self._h))
+
+    def alt_index(self, index: int, tag: int) -> int:
+        return int(lib().ccf_alt_index(self._h, index, tag))
+
+    def index_tag(self, key: int):
+        i, t = C.c_uint64(), C.c_uint32()
+        lib().ccf_index_tag(self._h, key, C.byref(i), C.byref(t))
+        return int(i.value), int(t.value)
+
+    def checksum(self) -> int:
+        return int(lib().ccf_checksum(self._h))
+
+    def _run(self, fn, keys, want_out):
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.zeros(k.size, dtype=np.uint8) if want_out else None
+        ok = int(fn(self._h, _ptr(k), k.size, _ptr(out)))
+        return (ok, out.astype(bool)) if want_out else ok
+
+    def add_many(self, keys, want_out=False):
+        return self._run(lib().ccf_add_many, keys, want_out)
+
+    def contain_many(self, keys, want_out=False):
+        return self._run(lib().ccf_contain_many, keys, want_out)
+
+    def delete_many(self, keys, want_out=False):
+        return self._run(lib().ccf_delete_many, keys, want_out)
 
 
 def table_tags(image: np.ndarray, cfg: Config) -> np.ndarray:
