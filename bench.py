@@ -19,6 +19,7 @@ HBM bytes per launch read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
 summary in profiles/hbm_traffic_2p28.md.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-load-sweep]
+                    [--cpu-cuckoo-log2 20,24]
 """
 from __future__ import annotations
 
@@ -96,7 +97,7 @@ def _events(torch):
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
-def cpu_baseline(n_ins, n_look, n_del, budget_s=25.0):
+def cpu_baseline(n_ins, n_look, n_del, budget_s=25.0, cpu_cuckoo_log2=(20, 24)):
     """The oracle's lock-free multi-threaded CAS insert / lookup / delete on the
     host cores, over a bounded sample: the same 2^28-slot table filled to the
     same 95% load, but only a 1/8 slice of the lookups and deletes."""
@@ -148,7 +149,8 @@ def cpu_baseline(n_ins, n_look, n_del, budget_s=25.0):
     # 52-155), on that harness's workload: insert n = 95% of the capacity,
     # look the same n keys up, delete them all; one thread
     cc = {}
-    for label, cap in (("capacity_2p20", 1 << 20), ("capacity_2p24", 1 << 24)):
+    for lg in cpu_cuckoo_log2:
+        label, cap = f"capacity_2p{lg}", 1 << lg
         g = oracle.CpuCuckoo(cap)
         n1 = int(cap * 0.95)
         k1 = oracle.gen_keys(SEED + 2, n1)
@@ -415,6 +417,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-cuckoo-log2", default="20,24",
+                    help="log2 capacities for the single-threaded cpu-cuckoo baseline (28 takes about a minute)")
     ap.add_argument("--no-load-sweep", action="store_true",
                     help="skip the insertMany / insertManySorted / lookup / delete sweep over load factors")
     ap.add_argument("--no-roofline", action="store_true", help="skip the measured gather/CAS roofline")
@@ -599,7 +603,8 @@ def main():
                 r["frac_of_measured_gather32"] = round(r["achieved"] / g, 3)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline")
-        result["cpu_baseline"] = cpu_baseline(n_ins, N_LOOKUP, n_del)
+        result["cpu_baseline"] = cpu_baseline(n_ins, N_LOOKUP, n_del,
+                                              cpu_cuckoo_log2=[int(x) for x in args.cpu_cuckoo_log2.split(",") if x])
     print(json.dumps(result), flush=True)
     return 0
 
