@@ -83,6 +83,23 @@ __device__ __forceinline__ T* windowPart(unsigned char* w, uint64_t off) {
 }
 }  // namespace detail
 
+/// A level's place in the window, passed by a level-by-level launch so that
+/// the kernel starts on its keys without first reading the window's tables
+/// (a chain of dependent loads of a buffer just copied in).  Graph launches
+/// pass valid = 0 and the kernel reads the same values from the window.
+struct LevelArgs {
+    const BatchOpDesc* ops;
+    const uint32_t* levelOps;
+    const uint32_t* chunkCum;
+    uint32_t* done;
+    uint32_t p0, p1;    // the level's levelOps positions [p0, p1)
+    uint32_t g0, g1;    // its chunks [g0, g1)
+    uint32_t op0;       // levelOps[p0]
+    uint32_t mutates;   // the level holds an insert or delete
+    uint32_t valid;
+    uint32_t pad;
+};
+
 /// One level: a grid-stride loop over the level's chunks (each workgroup
 /// finds its batch by binary search over the chunk counts), then the last
 /// workgroup to finish records results[op] = the op's filter occupancy for
@@ -91,39 +108,66 @@ __device__ __forceinline__ T* windowPart(unsigned char* w, uint64_t off) {
 /// workgroup 0 records them first and no completion count is kept.
 template <typename Config, uint32_t Chunk>
 __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* __restrict__ window, uint32_t level,
-                                                                unsigned long long* __restrict__ results) {
+                                                                unsigned long long* __restrict__ results,
+                                                                LevelArgs la) {
     using Ops = FilterOps<Config>;
     using Key = typename Config::KeyType;
-    const auto* hdr = reinterpret_cast<const WindowHeader*>(window);
-    const auto* ops = detail::windowPart<const BatchOpDesc>(window, hdr->opsOff);
-    const auto* levelOps = detail::windowPart<const uint32_t>(window, hdr->levelOpsOff);
-    const auto* chunkCum = detail::windowPart<const uint32_t>(window, hdr->chunkCumOff);
-    const auto* opStart = detail::windowPart<const uint32_t>(window, hdr->opStartOff);
-    auto* done = detail::windowPart<uint32_t>(window, hdr->doneOff);
-    const bool mutates = detail::windowPart<const uint32_t>(window, hdr->levelMutOff)[level] != 0;
-    const uint32_t p0 = opStart[level];
-    const uint32_t p1 = opStart[level + 1];
-    const uint32_t g1 = chunkCum[p1];
+    const BatchOpDesc* ops;
+    const uint32_t* levelOps;
+    const uint32_t* chunkCum;
+    uint32_t* done;
+    uint32_t p0, p1, g0, g1;
+    bool mutates;
+    if (la.valid) {
+        ops = la.ops;
+        levelOps = la.levelOps;
+        chunkCum = la.chunkCum;
+        done = la.done;
+        p0 = la.p0;
+        p1 = la.p1;
+        g0 = la.g0;
+        g1 = la.g1;
+        mutates = la.mutates != 0;
+    } else {
+        const auto* hdr = reinterpret_cast<const WindowHeader*>(window);
+        ops = detail::windowPart<const BatchOpDesc>(window, hdr->opsOff);
+        levelOps = detail::windowPart<const uint32_t>(window, hdr->levelOpsOff);
+        chunkCum = detail::windowPart<const uint32_t>(window, hdr->chunkCumOff);
+        done = detail::windowPart<uint32_t>(window, hdr->doneOff);
+        const auto* opStart = detail::windowPart<const uint32_t>(window, hdr->opStartOff);
+        p0 = opStart[level];
+        p1 = opStart[level + 1];
+        g0 = chunkCum[p0];
+        g1 = chunkCum[p1];
+        mutates = detail::windowPart<const uint32_t>(window, hdr->levelMutOff)[level] != 0;
+    }
+    // a level of one batch (most levels of a server's windows) needs no search
+    const bool single = p1 - p0 == 1;
+    const bool op0Known = single && la.valid;
     if (!mutates && blockIdx.x == 0) {
         for (uint32_t j = p0 + threadIdx.x; j < p1; j += kBatchBlock) {
-            const uint32_t op = levelOps[j];
+            const uint32_t op = op0Known ? la.op0 : levelOps[j];
             results[op] = __hip_atomic_load(ops[op].occupied, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
-    for (uint32_t g = chunkCum[p0] + blockIdx.x; g < g1; g += gridDim.x) {
+    for (uint32_t g = g0 + blockIdx.x; g < g1; g += gridDim.x) {
         // last position j in [p0, p1) with chunkCum[j] <= g (skips empty batches)
         uint32_t lo = p0;
-        uint32_t hi = p1 - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (chunkCum[mid] <= g) {
-                lo = mid;
-            } else {
-                hi = mid - 1;
+        uint32_t base = g0;
+        if (!single) {
+            uint32_t hi = p1 - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (chunkCum[mid] <= g) {
+                    lo = mid;
+                } else {
+                    hi = mid - 1;
+                }
             }
+            base = chunkCum[lo];
         }
-        const BatchOpDesc& o = ops[levelOps[lo]];
-        const uint64_t offset = static_cast<uint64_t>(g - chunkCum[lo]) * Chunk;
+        const BatchOpDesc& o = ops[op0Known ? la.op0 : levelOps[lo]];
+        const uint64_t offset = static_cast<uint64_t>(g - base) * Chunk;
         const uint32_t count = static_cast<uint32_t>(o.n - offset < Chunk ? o.n - offset : Chunk);
         DeviceView<Config> v;
         v.words = static_cast<typename DeviceView<Config>::Word*>(o.words);
@@ -168,7 +212,7 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* _
     // workgroups that own a chunk (block b < the level's chunk count) take
     // part; with no chunk at all, workgroup 0 snapshots alone.
     if (!mutates) return;
-    const uint32_t levelChunks = g1 - chunkCum[p0];
+    const uint32_t levelChunks = g1 - g0;
     const uint32_t parts = levelChunks < gridDim.x ? levelChunks : gridDim.x;
     if (parts == 0 ? blockIdx.x != 0 : blockIdx.x >= parts) return;
     __shared__ bool last;
@@ -181,7 +225,7 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* _
     if (last) {
         __threadfence();
         for (uint32_t j = p0 + threadIdx.x; j < p1; j += kBatchBlock) {
-            const uint32_t op = levelOps[j];
+            const uint32_t op = op0Known ? la.op0 : levelOps[j];
             results[op] = __hip_atomic_load(ops[op].occupied, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -246,13 +290,25 @@ class BatchExecutor {
         } else {
             // level by level, each with a grid no larger than its chunk count
             const uint32_t grid = levelGrid();
+            LevelArgs la{};
+            la.ops = reinterpret_cast<const BatchOpDesc*>(win + hdr_.opsOff);
+            la.levelOps = reinterpret_cast<const uint32_t*>(win + hdr_.levelOpsOff);
+            la.chunkCum = reinterpret_cast<const uint32_t*>(win + hdr_.chunkCumOff);
+            la.done = reinterpret_cast<uint32_t*>(win + hdr_.doneOff);
+            la.valid = 1;
             for (uint32_t l = 0; l < numLevels; ++l) {
-                const uint32_t chunks = chunkCum_[opStart_[l + 1]] - chunkCum_[opStart_[l]];
+                la.p0 = opStart_[l];
+                la.p1 = opStart_[l + 1];
+                la.g0 = chunkCum_[la.p0];
+                la.g1 = chunkCum_[la.p1];
+                la.op0 = levelOps_[la.p0];
+                la.mutates = levelMut_[l];
+                const uint32_t chunks = la.g1 - la.g0;
                 const uint32_t g = chunks == 0 ? 1u : (chunks < grid ? chunks : grid);
                 if (chunkKeys == kBatchChunk) {
-                    batchLevelKernel<Config, kBatchChunk><<<g, kBatchBlock, 0, stream>>>(win, l, results);
+                    batchLevelKernel<Config, kBatchChunk><<<g, kBatchBlock, 0, stream>>>(win, l, results, la);
                 } else {
-                    batchLevelKernel<Config, kBatchChunkLatency><<<g, kBatchBlock, 0, stream>>>(win, l, results);
+                    batchLevelKernel<Config, kBatchChunkLatency><<<g, kBatchBlock, 0, stream>>>(win, l, results, la);
                 }
             }
             if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -334,7 +390,8 @@ class BatchExecutor {
             if (ops[i].kind != static_cast<uint32_t>(BatchOpKind::Lookup)) levelMut_[level_[i]] = 1;
         }
 
-        WindowHeader h{};
+        WindowHeader& h = hdr_;
+        h = WindowHeader{};
         h.numOps = static_cast<uint32_t>(count);
         h.numLevels = numLevels;
         size_t off = align64(sizeof(WindowHeader));
@@ -420,7 +477,8 @@ class BatchExecutor {
         const uint32_t grid = levelGrid();
         if ((e = hipStreamBeginCapture(captureStream_, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
         for (uint32_t l = 0; l < numLevels; ++l) {
-            batchLevelKernel<Config, kBatchChunk><<<grid, kBatchBlock, 0, captureStream_>>>(win, l, results);
+            batchLevelKernel<Config, kBatchChunk><<<grid, kBatchBlock, 0, captureStream_>>>(win, l, results,
+                                                                                           LevelArgs{});
         }
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(captureStream_, &graph);
@@ -449,6 +507,7 @@ class BatchExecutor {
     std::unordered_map<const void*, uint32_t> seen_;
     std::vector<FilterLevelsPod> state_;
     std::vector<uint32_t> level_, opStart_, levelOps_, chunkCum_, cursor_, levelMut_;
+    WindowHeader hdr_{};  // the last planned window's header (its offsets into the device window)
     // graphs by level count
     hipStream_t captureStream_ = nullptr;
     hipGraphExec_t graphs_[kMaxGraphLevels + 1] = {};
