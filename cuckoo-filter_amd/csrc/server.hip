@@ -8,7 +8,10 @@
 // One stream runs every window, so windows execute in launch order and the
 // batch executor's per-window ordering gives arrival order per filter.
 // Two slots: one window runs on the device while the next one is formed,
-// staged and queued behind it.
+// staged and queued behind it.  Host-memory keys are copied into the slot's
+// pinned arena by several threads when a window holds many of them, and go
+// to the device on a copy stream, so that the upload of one window overlaps
+// the kernels of the one before it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +36,52 @@ namespace {
 
 constexpr int kSlots = 2;
 constexpr size_t kStageAlign = 256;
+constexpr size_t kParallelCopyBytes = 8u << 20;  // host copies at least this large use several threads
+constexpr unsigned kMaxCopyThreads = 8;
+
+struct CopyJob {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+
+// Copies every job; with enough bytes, splits the total evenly over threads
+// (each thread copies one contiguous share of the concatenated jobs).  Falls
+// back to one thread when threads cannot be started.
+void copyJobs(const std::vector<CopyJob>& jobs) {
+    size_t total = 0;
+    for (const CopyJob& j : jobs) total += j.bytes;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = total < kParallelCopyBytes ? 1u : std::min(kMaxCopyThreads, std::max(1u, hw / 2));
+    auto run = [&](size_t from, size_t to) {  // byte range of the concatenation
+        size_t at = 0;
+        for (const CopyJob& j : jobs) {
+            const size_t lo = std::max(from, at), hi = std::min(to, at + j.bytes);
+            if (lo < hi)
+                std::memcpy(static_cast<char*>(j.dst) + (lo - at), static_cast<const char*>(j.src) + (lo - at),
+                            hi - lo);
+            at += j.bytes;
+            if (at >= to) break;
+        }
+    };
+    if (nt == 1) {
+        run(0, total);
+        return;
+    }
+    std::vector<std::thread> helpers;
+    size_t started = 0;  // bytes handed to helper threads
+    try {
+        for (unsigned t = 1; t < nt; ++t) {
+            const size_t from = total * t / nt, to = total * (t + 1) / nt;
+            helpers.emplace_back(run, from, to);
+            started = to;
+        }
+    } catch (...) {
+    }
+    run(0, total / nt);  // this thread's share
+    if (started < total) run(std::max(started, total / nt), total);  // shares whose thread did not start
+    for (std::thread& h : helpers) h.join();
+}
 
 size_t alignUp(size_t x) { return (x + kStageAlign - 1) & ~(kStageAlign - 1); }
 
@@ -58,6 +107,7 @@ struct Slot {
     void* dArena = nullptr;  // device staging
     void* hArena = nullptr;  // pinned host staging (same layout)
     hipEvent_t done = nullptr;
+    hipEvent_t uploaded = nullptr;  // the window's keys are on the device (copy stream)
     // the window in this slot
     std::vector<Pending> reqs;
     std::vector<cuckoo::BatchOpDesc> descs;
@@ -76,6 +126,7 @@ struct cf_server {
     cf_server_options opt{};
     size_t arenaBytes = 0;  // per slot
     hipStream_t stream = nullptr;
+    hipStream_t copyStream = nullptr;  // host-to-device uploads of staged keys
     Slot slots[kSlots];
 
     std::mutex mu;
@@ -134,20 +185,27 @@ int cf_server::launchWindow(Slot& sl) {
     auto* h = static_cast<unsigned char*>(sl.hArena);
     auto* d = static_cast<unsigned char*>(sl.dArena);
     sl.descs.resize(sl.reqs.size());
+    std::vector<CopyJob> stage;
     for (size_t i = 0; i < sl.reqs.size(); ++i) {
         const cf_request& r = sl.reqs[i].req;
         const void* keys = r.keys;
         bool* out = r.output;
         if (r.flags & CF_REQUEST_HOST_MEMORY) {
             const size_t kb = static_cast<size_t>(r.n) * r.filter->cfg.key_bytes;
-            if (kb != 0) std::memcpy(h + sl.keyOff[i], r.keys, kb);
+            if (kb != 0) stage.push_back({h + sl.keyOff[i], r.keys, kb});
             keys = d + sl.keyOff[i];
             out = sl.reqs[i].stageOut != 0 ? reinterpret_cast<bool*>(d + sl.outOff[i]) : nullptr;
         }
         sl.descs[i] = describe(r.filter, r.op, keys, r.n, out);
     }
+    copyJobs(stage);
     hipError_t e = hipSuccess;
-    if (keyBytes != 0) e = hipMemcpyAsync(d, h, keyBytes, hipMemcpyHostToDevice, stream);
+    if (keyBytes != 0) {
+        // upload on the copy stream; the window's kernels wait for it on the main stream
+        e = hipMemcpyAsync(d, h, keyBytes, hipMemcpyHostToDevice, copyStream);
+        if (e == hipSuccess) e = hipEventRecord(sl.uploaded, copyStream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, sl.uploaded, 0);
+    }
     void** st = static_cast<void**>(executorFor(sl, sl.ops));
     // Windows differ from one to the next: level by level, each level's grid
     // sized to its work, and one key per thread, so that a small request's
@@ -163,7 +221,8 @@ int cf_server::launchWindow(Slot& sl) {
     sl.levels = e == hipSuccess ? sl.ops->batchLevels(sl.state) : 0;
     if (e != hipSuccess) {
         const int rc = hipFail(e, "launching a window");
-        (void)hipStreamSynchronize(stream);  // nothing of this window may still use the slot
+        (void)hipStreamSynchronize(copyStream);  // nothing of this window may still use the slot
+        (void)hipStreamSynchronize(stream);
         return rc;
     }
     return CF_OK;
@@ -230,9 +289,16 @@ void cf_server::completerLoop() {
         const unsigned long long* occ = status == CF_OK ? sl.ops->batchResults(sl.state) : nullptr;
         const auto* h = static_cast<const unsigned char*>(sl.hArena);
         std::vector<cf_request_result> res(sl.reqs.size());
+        if (status == CF_OK) {
+            std::vector<CopyJob> back;
+            for (size_t i = 0; i < sl.reqs.size(); ++i) {
+                const Pending& p = sl.reqs[i];
+                if (p.stageOut != 0 && p.req.n != 0) back.push_back({p.req.output, h + sl.outOff[i], p.req.n});
+            }
+            copyJobs(back);
+        }
         for (size_t i = 0; i < sl.reqs.size(); ++i) {
             const Pending& p = sl.reqs[i];
-            if (status == CF_OK && p.stageOut != 0) std::memcpy(p.req.output, h + sl.outOff[i], p.req.n);
             cf_request_result& r = res[i];
             r.status = status;
             r.window_levels = sl.levels;
@@ -324,7 +390,9 @@ void freeServer(cf_server* s) {
         if (sl.dArena != nullptr) (void)hipFree(sl.dArena);
         if (sl.hArena != nullptr) (void)hipHostFree(sl.hArena);
         if (sl.done != nullptr) (void)hipEventDestroy(sl.done);
+        if (sl.uploaded != nullptr) (void)hipEventDestroy(sl.uploaded);
     }
+    if (s->copyStream != nullptr) (void)hipStreamDestroy(s->copyStream);
     if (s->stream != nullptr) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -375,11 +443,13 @@ int cf_server_create(const cf_server_options* o, cf_server** out) {
     s->arenaBytes = std::max<size_t>(kStageAlign, (staging / kSlots) & ~(kStageAlign - 1));
     s->stats.staging_bytes = s->arenaBytes * kSlots;
     e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copyStream, hipStreamNonBlocking);
     for (int i = 0; i < kSlots && e == hipSuccess; ++i) {
         Slot& sl = s->slots[i];
         e = hipMalloc(&sl.dArena, s->arenaBytes);
         if (e == hipSuccess) e = hipHostMalloc(&sl.hArena, s->arenaBytes, hipHostMallocDefault);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.uploaded, hipEventDisableTiming);
         s->freeSlots.push_back(kSlots - 1 - i);
     }
     if (e != hipSuccess) {

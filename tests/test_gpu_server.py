@@ -247,3 +247,35 @@ def test_server_destroy_finishes_pending_requests(cf, oracle_mod):
     srv.close()  # returns once every queued request has run
     assert f.occupiedSlots() == keys.size
     assert all(bool(o.all().item()) for o in outs)
+
+
+def test_server_large_host_memory_windows(cf, oracle_mod):
+    """Requests of more than 8 MB of host-memory keys each, so that every
+    window holding one is staged by several threads and uploaded on the copy
+    stream: inserts then lookups through host memory give the plain calls'
+    occupancies and the oracle's lookup bits."""
+    O = oracle_mod
+    c = cf.make_config(eviction="bfs")
+    oc = O.make_config(**c.as_dict())
+    f = cf.CuckooFilter(1 << 24, config=c)
+    of = O.Filter(oc, 1 << 24)
+    n_each, parts = 3 << 19, 6  # 12 MiB of keys per request
+    keys = O.gen_keys(880, n_each * parts)
+    reqs = [(f, cf.OP_INSERT, keys[p * n_each:(p + 1) * n_each].copy(), None, True) for p in range(parts)]
+    with cf.Server() as srv:
+        res = srv.replay(srv.make_requests(reqs), np.zeros(parts))
+        assert (res["status"] == 0).all()
+        np.testing.assert_array_equal(res["occupied"], np.arange(1, parts + 1, dtype=np.uint64) * n_each)
+        # the oracle holds the served image; lookups through host memory must match it bit for bit
+        of.load_table(f.exportTable(), f.occupiedSlots())
+        q = np.concatenate([keys[::3], O.gen_keys(881, keys.size // 3)])  # 8 MiB per lookup request
+        chunks = np.array_split(q, parts)
+        outs = [np.zeros(ch.size, dtype=bool) for ch in chunks]
+        lreqs = [(f, cf.OP_LOOKUP, ch, o, True) for ch, o in zip(chunks, outs)]
+        res = srv.replay(srv.make_requests(lreqs), np.zeros(parts))
+        assert (res["status"] == 0).all()
+    got = np.concatenate(outs)
+    np.testing.assert_array_equal(got, of.contains_many(q))
+    assert got[: keys[::3].size].all()
+    plain = f.containsMany(cf.keys_to_device(q)).cpu().numpy()
+    np.testing.assert_array_equal(got, plain)
