@@ -358,7 +358,9 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
     def host_burst(srv, count=2000):
         """The first `count` requests with keys and results in pageable host
         memory, replayed as one burst from the same starting states: the rate
-        includes the staging copies and PCIe in both directions."""
+        includes the staging copies and PCIe in both directions.  Result
+        buffers are written once before the burst, as a client reusing its
+        buffers would have them, so first-touch page faults stay out of it."""
         sub = reqs[:count].copy()
         keep = []
         for j in range(count):
@@ -370,12 +372,15 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
             sub["flags"][j] = cf.CF_REQUEST_HOST_MEMORY
             sub["keys"][j] = hk.ctypes.data
             if int(sub["op"][j]) == 1:
-                o = np.zeros(n, dtype=bool)
+                o = np.empty(n, dtype=bool)
+                o.fill(False)
                 keep.append(o)
                 sub["output"][j] = o.ctypes.data
             else:
                 sub["output"][j] = 0
         keys_total = int(sub["n"].sum())
+        restore()
+        srv.replay(sub, np.zeros(count))  # warm-up: the server's arenas and executors
         restore()
         res = srv.replay(sub, np.zeros(count))
         assert (res["status"] == 0).all(), "host-memory request failed"
@@ -397,7 +402,6 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
         run(srv, np.zeros(n_requests))  # warm-up: plans, graphs, arenas
         sat = run(srv, np.zeros(n_requests))
         out["saturated"] = sat
-        out["host_memory_burst"] = host_burst(srv)
         peak = sat["mkeys_s"] * 1e6
         mean_keys = total_keys / n_requests
         for frac in loads:
@@ -405,6 +409,8 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
             arr = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
             arr -= arr[0]
             out[f"load_{int(frac * 100)}pct"] = dict(run(srv, arr), offered_mkeys_s=round(frac * peak / 1e6, 1))
+    with cf.Server(window_slots=3) as srv3:  # a third slot overlaps host copies with uploads
+        out["host_memory_burst"] = dict(host_burst(srv3), window_slots=3)
     for x in filters + snapshots:
         x.close()
     del kmers, pools, lpools, ring

@@ -7,8 +7,10 @@
 // order, copies host-memory results out and publishes per-request results.
 // One stream runs every window, so windows execute in launch order and the
 // batch executor's per-window ordering gives arrival order per filter.
-// Two slots: one window runs on the device while the next one is formed,
-// staged and queued behind it.  Host-memory keys are copied into the slot's
+// Window slots (two by default, window_slots in the options): one window runs
+// on the device while the next ones are formed, staged and queued behind it.
+// A third slot lets the host copy of one window overlap the upload of the one
+// before it, which pays when requests come from host memory.  Host-memory keys are copied into the slot's
 // pinned arena by several threads when a window holds many of them, and go
 // to the device on a copy stream, so that the upload of one window overlaps
 // the kernels of the one before it.
@@ -34,7 +36,7 @@ using namespace cfabi::internal;
 
 namespace {
 
-constexpr int kSlots = 2;
+constexpr int kMaxSlots = 8;  // window slots: one window runs while the others are formed and staged
 constexpr size_t kStageAlign = 256;
 constexpr size_t kParallelCopyBytes = 8u << 20;  // host copies at least this large use several threads
 constexpr unsigned kMaxCopyThreads = 8;
@@ -127,7 +129,8 @@ struct cf_server {
     size_t arenaBytes = 0;  // per slot
     hipStream_t stream = nullptr;
     hipStream_t copyStream = nullptr;  // host-to-device uploads of staged keys
-    Slot slots[kSlots];
+    Slot slots[kMaxSlots];
+    int numSlots = 2;
 
     std::mutex mu;
     std::condition_variable cvWork;      // launcher: queue or free slot changed
@@ -411,6 +414,8 @@ int cf_server_options_init(cf_server_options* o) {
     o->max_window_requests = 4096;
     o->max_window_keys = 1ull << 22;
     o->staging_bytes = 0;
+    o->window_slots = 2;
+    o->reserved = 0;
     return CF_OK;
 }
 
@@ -425,6 +430,9 @@ int cf_server_create(const cf_server_options* o, cf_server** out) {
     }
     if (opt.max_window_requests == 0 || opt.max_window_keys == 0)
         return fail(CF_ERR_INVALID, "window limits must be positive");
+    if (opt.window_slots < 2 || opt.window_slots > kMaxSlots)
+        return fail(CF_ERR_INVALID, "window_slots must be 2..%d (got %u)", kMaxSlots, opt.window_slots);
+    if (opt.reserved != 0) return fail(CF_ERR_INVALID, "reserved options field must be 0");
     auto* s = new (std::nothrow) cf_server();
     if (s == nullptr) return fail(CF_ERR_ALLOC, "host allocation failed");
     s->opt = opt;
@@ -440,17 +448,18 @@ int cf_server_create(const cf_server_options* o, cf_server** out) {
         }
         staging = std::min<size_t>(512ull << 20, freeB / 16);
     }
-    s->arenaBytes = std::max<size_t>(kStageAlign, (staging / kSlots) & ~(kStageAlign - 1));
-    s->stats.staging_bytes = s->arenaBytes * kSlots;
+    s->numSlots = static_cast<int>(opt.window_slots);
+    s->arenaBytes = std::max<size_t>(kStageAlign, (staging / s->numSlots) & ~(kStageAlign - 1));
+    s->stats.staging_bytes = s->arenaBytes * s->numSlots;
     e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copyStream, hipStreamNonBlocking);
-    for (int i = 0; i < kSlots && e == hipSuccess; ++i) {
+    for (int i = 0; i < s->numSlots && e == hipSuccess; ++i) {
         Slot& sl = s->slots[i];
         e = hipMalloc(&sl.dArena, s->arenaBytes);
         if (e == hipSuccess) e = hipHostMalloc(&sl.hArena, s->arenaBytes, hipHostMallocDefault);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.uploaded, hipEventDisableTiming);
-        s->freeSlots.push_back(kSlots - 1 - i);
+        s->freeSlots.push_back(s->numSlots - 1 - i);
     }
     if (e != hipSuccess) {
         const int rc = hipFail(e, "server set-up");

@@ -55,11 +55,11 @@ REQUEST_DTYPE = np.dtype([("filter", "u8"), ("op", "u4"), ("flags", "u4"), ("key
 RESULT_DTYPE = np.dtype([("status", "i4"), ("window_levels", "u4"), ("window", "u8"), ("occupied", "u8"),
                          ("arrival_ns", "u8"), ("start_ns", "u8"), ("done_ns", "u8")])
 SERVER_OPTIONS_DTYPE = np.dtype([("device", "i4"), ("max_window_requests", "u4"), ("max_window_keys", "u8"),
-                                 ("staging_bytes", "u8")])
+                                 ("staging_bytes", "u8"), ("window_slots", "u4"), ("reserved", "u4")])
 SERVER_STATS_DTYPE = np.dtype([(k, "u8") for k in ("windows", "requests", "keys", "levels", "staging_bytes",
                                                     "max_queue")])
 assert REQUEST_DTYPE.itemsize == 40 and RESULT_DTYPE.itemsize == 48
-assert SERVER_OPTIONS_DTYPE.itemsize == 24 and SERVER_STATS_DTYPE.itemsize == 48
+assert SERVER_OPTIONS_DTYPE.itemsize == 32 and SERVER_STATS_DTYPE.itemsize == 48
 
 
 class cf_config(C.Structure):
@@ -457,7 +457,8 @@ class Server:
     or numpy arrays with host=True (staged through the server's device arena).
     Results equal running the requests one by one in arrival order."""
 
-    def __init__(self, device=None, max_window_requests=None, max_window_keys=None, staging_bytes=None):
+    def __init__(self, device=None, max_window_requests=None, max_window_keys=None, staging_bytes=None,
+                 window_slots=None):
         o = np.zeros(1, SERVER_OPTIONS_DTYPE)
         _check(lib.cf_server_options_init(o.ctypes.data))
         if device is not None:
@@ -468,6 +469,8 @@ class Server:
             o["max_window_keys"] = max_window_keys
         if staging_bytes is not None:
             o["staging_bytes"] = staging_bytes
+        if window_slots is not None:
+            o["window_slots"] = window_slots
         h = C.c_void_p()
         _check(lib.cf_server_create(o.ctypes.data, C.byref(h)))
         self._h = h

@@ -204,6 +204,9 @@ typedef struct cf_server_options {
     uint64_t max_window_keys;     /* default 1 << 22 (a larger single request runs alone) */
     uint64_t staging_bytes;       /* device staging arena for host-memory requests;
                                      0 = min(512 MiB, 1/16 of free HBM at creation) */
+    uint32_t window_slots;        /* windows formed ahead: 2..8, default 2; 3 overlaps the
+                                     staging of host-memory windows with their uploads */
+    uint32_t reserved;            /* must be 0 */
 } cf_server_options;
 
 typedef struct cf_request {

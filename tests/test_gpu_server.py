@@ -249,7 +249,8 @@ def test_server_destroy_finishes_pending_requests(cf, oracle_mod):
     assert all(bool(o.all().item()) for o in outs)
 
 
-def test_server_large_host_memory_windows(cf, oracle_mod):
+@pytest.mark.parametrize("slots", [2, 3])
+def test_server_large_host_memory_windows(cf, oracle_mod, slots):
     """Requests of more than 8 MB of host-memory keys each, so that every
     window holding one is staged by several threads and uploaded on the copy
     stream: inserts then lookups through host memory give the plain calls'
@@ -262,7 +263,7 @@ def test_server_large_host_memory_windows(cf, oracle_mod):
     n_each, parts = 3 << 19, 6  # 12 MiB of keys per request
     keys = O.gen_keys(880, n_each * parts)
     reqs = [(f, cf.OP_INSERT, keys[p * n_each:(p + 1) * n_each].copy(), None, True) for p in range(parts)]
-    with cf.Server() as srv:
+    with cf.Server(window_slots=slots) as srv:
         res = srv.replay(srv.make_requests(reqs), np.zeros(parts))
         assert (res["status"] == 0).all()
         np.testing.assert_array_equal(res["occupied"], np.arange(1, parts + 1, dtype=np.uint64) * n_each)
