@@ -14,7 +14,9 @@
 // In a level with an insert or delete, the last workgroup to finish records
 // every batch's filter occupancy, which is exactly the value the plain
 // insertMany / deleteMany call would have returned; a level of lookups only
-// changes no counter, so its first workgroup records them at once.
+// changes no counter, so its first workgroup records them at once, and also
+// those of the level before it when that level mutated: the mutating level
+// then keeps no completion count.
 //
 // A window is one device buffer: a header, the batch descriptors, the
 // level tables and per-level completion counters, uploaded with one copy
@@ -73,7 +75,7 @@ struct alignas(64) WindowHeader {
     uint64_t chunkCumOff;  // uint32_t[numOps + 1]: chunks before each levelOps position
     uint64_t opStartOff;   // uint32_t[numLevels + 1]: first levelOps position of each level
     uint64_t doneOff;      // uint32_t[numLevels]: workgroups finished per level (uploaded as zero)
-    uint64_t levelMutOff;  // uint32_t[numLevels]: 1 if the level holds an insert or delete
+    uint64_t levelModeOff;  // uint32_t[numLevels]: a LevelMode
 };
 
 namespace detail {
@@ -82,6 +84,13 @@ __device__ __forceinline__ T* windowPart(unsigned char* w, uint64_t off) {
     return reinterpret_cast<T*>(w + off);
 }
 }  // namespace detail
+
+/// Who records a level's occupancies.
+enum LevelMode : uint32_t {
+    kLookupsOnly = 0,     // workgroup 0, as the level starts (and the previous level's, if deferred)
+    kMutatesRecords = 1,  // the level mutates; its last workgroup records them
+    kMutatesDeferred = 2  // the level mutates; the next level (lookups only) records them
+};
 
 /// A level's place in the window, passed by a level-by-level launch so that
 /// the kernel starts on its keys without first reading the window's tables
@@ -95,9 +104,9 @@ struct LevelArgs {
     uint32_t p0, p1;    // the level's levelOps positions [p0, p1)
     uint32_t g0, g1;    // its chunks [g0, g1)
     uint32_t op0;       // levelOps[p0]
-    uint32_t mutates;   // the level holds an insert or delete
+    uint32_t mode;      // LevelMode
+    uint32_t r0;        // first levelOps position workgroup 0 records (p0, or the previous level's start)
     uint32_t valid;
-    uint32_t pad;
 };
 
 /// One level: a grid-stride loop over the level's chunks (each workgroup
@@ -105,7 +114,8 @@ struct LevelArgs {
 /// workgroup to finish records results[op] = the op's filter occupancy for
 /// every op of the level.  A level of lookups only leaves every counter as
 /// the previous level (an earlier kernel on the stream) left it, so
-/// workgroup 0 records them first and no completion count is kept.
+/// workgroup 0 records them first, together with the previous level's when
+/// that level deferred them, and no completion count is kept.
 template <typename Config, uint32_t Chunk>
 __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* __restrict__ window, uint32_t level,
                                                                 unsigned long long* __restrict__ results,
@@ -116,8 +126,7 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* _
     const uint32_t* levelOps;
     const uint32_t* chunkCum;
     uint32_t* done;
-    uint32_t p0, p1, g0, g1;
-    bool mutates;
+    uint32_t p0, p1, g0, g1, mode, r0;
     if (la.valid) {
         ops = la.ops;
         levelOps = la.levelOps;
@@ -127,7 +136,8 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* _
         p1 = la.p1;
         g0 = la.g0;
         g1 = la.g1;
-        mutates = la.mutates != 0;
+        mode = la.mode;
+        r0 = la.r0;
     } else {
         const auto* hdr = reinterpret_cast<const WindowHeader*>(window);
         ops = detail::windowPart<const BatchOpDesc>(window, hdr->opsOff);
@@ -139,14 +149,16 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* _
         p1 = opStart[level + 1];
         g0 = chunkCum[p0];
         g1 = chunkCum[p1];
-        mutates = detail::windowPart<const uint32_t>(window, hdr->levelMutOff)[level] != 0;
+        const auto* levelMode = detail::windowPart<const uint32_t>(window, hdr->levelModeOff);
+        mode = levelMode[level];
+        r0 = level > 0 && levelMode[level - 1] == kMutatesDeferred ? opStart[level - 1] : p0;
     }
     // a level of one batch (most levels of a server's windows) needs no search
     const bool single = p1 - p0 == 1;
     const bool op0Known = single && la.valid;
-    if (!mutates && blockIdx.x == 0) {
-        for (uint32_t j = p0 + threadIdx.x; j < p1; j += kBatchBlock) {
-            const uint32_t op = op0Known ? la.op0 : levelOps[j];
+    if (mode == kLookupsOnly && blockIdx.x == 0) {
+        for (uint32_t j = r0 + threadIdx.x; j < p1; j += kBatchBlock) {
+            const uint32_t op = op0Known && j == p0 ? la.op0 : levelOps[j];
             results[op] = __hip_atomic_load(ops[op].occupied, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -211,7 +223,7 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* _
     // workgroup's counter updates precede its increment of done[level].  Only
     // workgroups that own a chunk (block b < the level's chunk count) take
     // part; with no chunk at all, workgroup 0 snapshots alone.
-    if (!mutates) return;
+    if (mode != kMutatesRecords) return;
     const uint32_t levelChunks = g1 - g0;
     const uint32_t parts = levelChunks < gridDim.x ? levelChunks : gridDim.x;
     if (parts == 0 ? blockIdx.x != 0 : blockIdx.x >= parts) return;
@@ -302,7 +314,8 @@ class BatchExecutor {
                 la.g0 = chunkCum_[la.p0];
                 la.g1 = chunkCum_[la.p1];
                 la.op0 = levelOps_[la.p0];
-                la.mutates = levelMut_[l];
+                la.mode = levelMode_[l];
+                la.r0 = l > 0 && levelMode_[l - 1] == kMutatesDeferred ? opStart_[l - 1] : la.p0;
                 const uint32_t chunks = la.g1 - la.g0;
                 const uint32_t g = chunks == 0 ? 1u : (chunks < grid ? chunks : grid);
                 if (chunkKeys == kBatchChunk) {
@@ -385,9 +398,12 @@ class BatchExecutor {
             }
             chunkCum_[j + 1] = static_cast<uint32_t>(chunks);
         }
-        levelMut_.assign(numLevels, 0);
+        levelMode_.assign(numLevels, kLookupsOnly);
         for (size_t i = 0; i < count; ++i) {
-            if (ops[i].kind != static_cast<uint32_t>(BatchOpKind::Lookup)) levelMut_[level_[i]] = 1;
+            if (ops[i].kind != static_cast<uint32_t>(BatchOpKind::Lookup)) levelMode_[level_[i]] = kMutatesRecords;
+        }
+        for (uint32_t l = 0; l + 1 < numLevels; ++l) {
+            if (levelMode_[l] == kMutatesRecords && levelMode_[l + 1] == kLookupsOnly) levelMode_[l] = kMutatesDeferred;
         }
 
         WindowHeader& h = hdr_;
@@ -405,7 +421,7 @@ class BatchExecutor {
         off = align64(off + (numLevels + 1) * sizeof(uint32_t));
         h.doneOff = off;
         off = align64(off + numLevels * sizeof(uint32_t));
-        h.levelMutOff = off;
+        h.levelModeOff = off;
         off = align64(off + numLevels * sizeof(uint32_t));
         staged_ = off;
         stageError_ = growPinned(&hStage_, &capStage_, staged_);
@@ -420,7 +436,7 @@ class BatchExecutor {
         std::memcpy(b + h.chunkCumOff, chunkCum_.data(), (count + 1) * sizeof(uint32_t));
         std::memcpy(b + h.opStartOff, opStart_.data(), (numLevels + 1) * sizeof(uint32_t));
         std::memset(b + h.doneOff, 0, numLevels * sizeof(uint32_t));
-        std::memcpy(b + h.levelMutOff, levelMut_.data(), numLevels * sizeof(uint32_t));
+        std::memcpy(b + h.levelModeOff, levelMode_.data(), numLevels * sizeof(uint32_t));
         return numLevels;
     }
 
@@ -470,7 +486,8 @@ class BatchExecutor {
 
     hipError_t capture(uint32_t numLevels) {
         hipError_t e;
-        if (captureStream_ == nullptr && (e = hipStreamCreateWithFlags(&captureStream_, hipStreamNonBlocking)) != hipSuccess)
+        if (captureStream_ == nullptr &&
+            (e = hipStreamCreateWithFlags(&captureStream_, hipStreamNonBlocking)) != hipSuccess)
             return e;
         auto* win = static_cast<unsigned char*>(dWin_);
         auto* results = static_cast<unsigned long long*>(dResults_);
@@ -506,7 +523,7 @@ class BatchExecutor {
     // planning scratch (host)
     std::unordered_map<const void*, uint32_t> seen_;
     std::vector<FilterLevelsPod> state_;
-    std::vector<uint32_t> level_, opStart_, levelOps_, chunkCum_, cursor_, levelMut_;
+    std::vector<uint32_t> level_, opStart_, levelOps_, chunkCum_, cursor_, levelMode_;
     WindowHeader hdr_{};  // the last planned window's header (its offsets into the device window)
     // graphs by level count
     hipStream_t captureStream_ = nullptr;
