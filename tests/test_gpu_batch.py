@@ -110,6 +110,48 @@ def test_batch_keeps_per_filter_order(cf, oracle_mod):
     b.close()
 
 
+@pytest.mark.parametrize("chunk_sizes", [[3000] * 11, [1, 700, 5000, 2, 0, 1300, 1, 9000, 4, 60, 513, 511]])
+def test_batch_consecutive_inserts_report_sequential_occupancies(cf, oracle_mod, chunk_sizes):
+    """Runs of inserts on one filter, some longer than eight, broken by a
+    lookup or a delete, with a second filter and empty inserts between: each
+    insert reports the occupancy after itself.  Without insert failures
+    (under 50% load) every reported occupancy equals the plain calls'
+    exactly, and the lookup sees every insert before it."""
+    import torch
+
+    O = oracle_mod
+    f = cf.CuckooFilter(1 << 18, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    g = cf.CuckooFilter(1 << 18, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    pf = cf.CuckooFilter(1 << 18, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    pg = cf.CuckooFilter(1 << 18, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    keys = O.gen_keys(901, sum(chunk_sizes) + 1)
+    parts, at = [], 0
+    for n in chunk_sizes:
+        parts.append(keys[at:at + n])
+        at += n
+    plan = [(0, 0, p) for p in parts[:9]]                       # nine inserts in a row
+    plan += [(0, 1, parts[0]), (1, 0, parts[9]), (0, 0, parts[9])]  # a lookup breaks the run; g in between
+    plan += [(0, 0, parts[10]), (0, 2, parts[1]), (0, 0, parts[-1][:1])]  # insert, delete, insert
+    def run(fs):
+        ops, outs = [], []
+        for i, kind, k in plan:
+            dk = _dev(cf, k) if k.size else torch.zeros(0, dtype=torch.int64, device="cuda")
+            out = torch.zeros(max(k.size, 1), dtype=torch.bool, device="cuda") if kind else None
+            ops.append((fs[i], kind, dk, out))
+            outs.append(out)
+        return ops, outs
+    ops_b, outs_b = run([f, g])
+    ops_p, outs_p = run([pf, pg])
+    occ_p = _run_plain(ops_p)
+    b = cf.Batch()
+    occ_b = b.execute(ops_b)
+    b.close()
+    np.testing.assert_array_equal(occ_b, occ_p)
+    assert outs_b[9][: parts[0].size].all().item()  # the lookup sees every insert before it
+    assert f.occupiedSlots() == pf.occupiedSlots() and g.occupiedSlots() == pg.occupiedSlots()
+    assert_table_count(f, f.occupiedSlots(), f.capacity())
+
+
 def test_batch_mixed_window_matches_plain(cf, oracle_mod):
     import torch
 
