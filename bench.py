@@ -487,6 +487,10 @@ def main():
         step(True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # the last step's lookups (on the filled table): first half inserted keys, second half never inserted
+    half = N_LOOKUP // 2
+    checks["positives_found"] = int(out[:half].sum().item())
+    checks["false_positives"] = int(out[half:].sum().item())
 
     ms_step = wall * 1e3 / args.steps
     keys_per_step = n_ins + N_LOOKUP + n_del
@@ -580,7 +584,11 @@ def main():
         "roofline": roofline,
         "roofline_insert": roofline_insert,
         "checks": {"occupied_after_insert": checks.get("occupied"), "expected": n_ins,
-                   "occupied_after_delete": checks.get("remaining")},
+                   "occupied_after_delete": checks.get("remaining"),
+                   "lookup_positives_found": checks.get("positives_found"), "lookup_positives": N_LOOKUP // 2,
+                   "false_positive_rate": round(checks.get("false_positives", 0) / (N_LOOKUP // 2), 7),
+                   # 2 * bucketSize * alpha / 2^bitsPerTag (src/main.cu:114-115)
+                   "fpr_closed_form": round(2 * 16 * (checks.get("occupied") or 0) / CAPACITY / 2**16, 7)},
     }
     if load_sweep is not None:
         result["load_sweep"] = load_sweep
