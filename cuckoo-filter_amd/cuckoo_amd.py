@@ -455,7 +455,9 @@ class Server:
 
     A request is (filter, op, keys, out, host): keys / out are device tensors,
     or numpy arrays with host=True (staged through the server's device arena).
-    Results equal running the requests one by one in arrival order."""
+    Results equal running the requests one by one in arrival order.
+    window_slots (2..8, default 2) is how many windows are formed ahead of the
+    running one; 3 lets host-memory staging overlap the uploads."""
 
     def __init__(self, device=None, max_window_requests=None, max_window_keys=None, staging_bytes=None,
                  window_slots=None):
