@@ -47,42 +47,98 @@ struct CopyJob {
     size_t bytes;
 };
 
-// Copies every job; with enough bytes, splits the total evenly over threads
-// (each thread copies one contiguous share of the concatenated jobs).  Falls
-// back to one thread when threads cannot be started.
-void copyJobs(const std::vector<CopyJob>& jobs) {
-    size_t total = 0;
-    for (const CopyJob& j : jobs) total += j.bytes;
+// Copies byte range [from, to) of the concatenation of jobs.
+void copyRange(const std::vector<CopyJob>& jobs, size_t from, size_t to) {
+    size_t at = 0;
+    for (const CopyJob& j : jobs) {
+        const size_t lo = std::max(from, at), hi = std::min(to, at + j.bytes);
+        if (lo < hi)
+            std::memcpy(static_cast<char*>(j.dst) + (lo - at), static_cast<const char*>(j.src) + (lo - at), hi - lo);
+        at += j.bytes;
+        if (at >= to) break;
+    }
+}
+
+// Helper threads, started once, that split large host copies with the
+// calling thread: each copies one contiguous share of the concatenated jobs.
+// One pool serves one calling thread (run() is not reentrant).  Without
+// helpers (none could be started) run() copies alone.
+class CopyPool {
+   public:
+    explicit CopyPool(unsigned helpers) {
+        try {
+            for (unsigned k = 1; k <= helpers; ++k) threads_.emplace_back([this, k] { loop(k); });
+        } catch (...) {
+            stopAll();  // keep none rather than a partial set with gaps in the shares
+        }
+    }
+    CopyPool(const CopyPool&) = delete;
+    CopyPool& operator=(const CopyPool&) = delete;
+    ~CopyPool() { stopAll(); }
+
+    void run(const std::vector<CopyJob>& jobs) {
+        size_t total = 0;
+        for (const CopyJob& j : jobs) total += j.bytes;
+        const unsigned parts = total < kParallelCopyBytes ? 1u : static_cast<unsigned>(threads_.size()) + 1u;
+        if (parts == 1) {
+            copyRange(jobs, 0, total);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            jobs_ = &jobs;
+            total_ = total;
+            parts_ = parts;
+            pending_ = parts - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        copyRange(jobs, 0, total / parts);  // share 0
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+        jobs_ = nullptr;
+    }
+
+   private:
+    void loop(unsigned k) {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            const std::vector<CopyJob>* jobs = jobs_;
+            const size_t from = total_ * k / parts_, to = total_ * (k + 1) / parts_;
+            lk.unlock();
+            copyRange(*jobs, from, to);
+            lk.lock();
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    void stopAll() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread& t : threads_) t.join();
+        threads_.clear();
+    }
+
+    std::vector<std::thread> threads_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::vector<CopyJob>* jobs_ = nullptr;
+    size_t total_ = 0;
+    unsigned parts_ = 1;
+    unsigned pending_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+unsigned copyHelpers() {
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const unsigned nt = total < kParallelCopyBytes ? 1u : std::min(kMaxCopyThreads, std::max(1u, hw / 2));
-    auto run = [&](size_t from, size_t to) {  // byte range of the concatenation
-        size_t at = 0;
-        for (const CopyJob& j : jobs) {
-            const size_t lo = std::max(from, at), hi = std::min(to, at + j.bytes);
-            if (lo < hi)
-                std::memcpy(static_cast<char*>(j.dst) + (lo - at), static_cast<const char*>(j.src) + (lo - at),
-                            hi - lo);
-            at += j.bytes;
-            if (at >= to) break;
-        }
-    };
-    if (nt == 1) {
-        run(0, total);
-        return;
-    }
-    std::vector<std::thread> helpers;
-    size_t started = 0;  // bytes handed to helper threads
-    try {
-        for (unsigned t = 1; t < nt; ++t) {
-            const size_t from = total * t / nt, to = total * (t + 1) / nt;
-            helpers.emplace_back(run, from, to);
-            started = to;
-        }
-    } catch (...) {
-    }
-    run(0, total / nt);  // this thread's share
-    if (started < total) run(std::max(started, total / nt), total);  // shares whose thread did not start
-    for (std::thread& h : helpers) h.join();
+    return std::min(kMaxCopyThreads, std::max(1u, hw / 2)) - 1;
 }
 
 size_t alignUp(size_t x) { return (x + kStageAlign - 1) & ~(kStageAlign - 1); }
@@ -149,6 +205,8 @@ struct cf_server {
     bool launcherDone = false;
     cf_server_stats stats{};
 
+    CopyPool launcherCopies{copyHelpers()};   // stages keys into the pinned arena
+    CopyPool completerCopies{copyHelpers()};  // copies results out of it
     std::thread launcher;
     std::thread completer;
 
@@ -201,7 +259,7 @@ int cf_server::launchWindow(Slot& sl) {
         }
         sl.descs[i] = describe(r.filter, r.op, keys, r.n, out);
     }
-    copyJobs(stage);
+    launcherCopies.run(stage);
     hipError_t e = hipSuccess;
     if (keyBytes != 0) {
         // upload on the copy stream; the window's kernels wait for it on the main stream
@@ -298,7 +356,7 @@ void cf_server::completerLoop() {
                 const Pending& p = sl.reqs[i];
                 if (p.stageOut != 0 && p.req.n != 0) back.push_back({p.req.output, h + sl.outOff[i], p.req.n});
             }
-            copyJobs(back);
+            completerCopies.run(back);
         }
         for (size_t i = 0; i < sl.reqs.size(); ++i) {
             const Pending& p = sl.reqs[i];
