@@ -10,10 +10,10 @@
 // Window slots (two by default, window_slots in the options): one window runs
 // on the device while the next ones are formed, staged and queued behind it.
 // A third slot lets the host copy of one window overlap the upload of the one
-// before it, which pays when requests come from host memory.  Host-memory keys are copied into the slot's
-// pinned arena by several threads when a window holds many of them, and go
-// to the device on a copy stream, so that the upload of one window overlaps
-// the kernels of the one before it.
+// before it, which pays when requests come from host memory.  Host-memory keys
+// are copied into the slot's pinned arena by a pool of helper threads when a
+// window holds many of them, and go to the device on a copy stream, so that
+// the upload of one window overlaps the kernels of the one before it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
