@@ -536,16 +536,29 @@ struct BinnedInsert {
             EntrySource<Config, SortedSemantics> src{keys + off, v.numBuckets};
             const size_t lds = binLdsBytes((size_t(1) << l2s) * kBucketBytes, sizeof(BinRetryOf<Config>));
             if (twoLevel(sliceBits) && !twoLevelPacked(l2s, sliceBits)) {
-                // whole 64-bit entries: a holds the first pass's output, b the
-                // second's; then the lists as below
+                // the first pass keeps whole 64-bit entries in a; the second
+                // writes b, in 32 bits (the bucket's place in its slice beside
+                // the tag) where they fit, else whole entries; then the lists
+                // as below
                 const int d2 = sliceBits - radix::kDigitBits;
                 zeroCountersKernel<unsigned long long><<<1, 64, 0, s>>>(counters);
-                e = radix::partitionTwoLevel<uint64_t>(src, m, 32 + l2s + d2, radix::Identity<uint64_t>{}, 32 + l2s,
-                                                       uint32_t(1) << d2, radix::Identity<uint64_t>{}, a, b, scratch,
-                                                       start, static_cast<uint32_t>(slices), s);
-                if (e != hipSuccess) return e;
-                binInsertKernel<Config, uint64_t><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(
-                    b, start, l2s, v, a, counters);
+                if (l2s + kTagBits <= 32) {
+                    const PackBelowDigit pack2{(uint64_t(1) << l2s) - 1, kTagBits};
+                    uint32_t* b32 = reinterpret_cast<uint32_t*>(b);
+                    e = radix::partitionTwoLevel<uint64_t>(src, m, 32 + l2s + d2, radix::Identity<uint64_t>{},
+                                                           32 + l2s, uint32_t(1) << d2, pack2, a, b32, scratch,
+                                                           start, static_cast<uint32_t>(slices), s);
+                    if (e != hipSuccess) return e;
+                    binInsertKernel<Config, uint32_t><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(
+                        b32, start, l2s, v, a, counters);
+                } else {
+                    e = radix::partitionTwoLevel<uint64_t>(src, m, 32 + l2s + d2, radix::Identity<uint64_t>{},
+                                                           32 + l2s, uint32_t(1) << d2, radix::Identity<uint64_t>{},
+                                                           a, b, scratch, start, static_cast<uint32_t>(slices), s);
+                    if (e != hipSuccess) return e;
+                    binInsertKernel<Config, uint64_t><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(
+                        b, start, l2s, v, a, counters);
+                }
                 e = runStages(v, m, block, s, counters, a, b);
                 if (e != hipSuccess) return e;
                 continue;

@@ -277,13 +277,18 @@ def test_batch_insert_invariants_and_binomial_parity(cf, oracle_mod, kw, mode, m
 # Tables of 2^9 slices and more split the slice index into two digits.  Where
 # the bucket bits below the first digit fit beside the tag in 32 bits, the
 # binned insert partitions in two levels (MSD) with 32-bit entries; otherwise
-# (32-bit tags) it keeps the LSD passes with 64-bit entries.
+# the first level keeps whole 64-bit entries, and the second writes 32-bit
+# entries where the bucket's place in its slice fits beside the tag (16-bit
+# tags in large tables) or 64-bit ones (32-bit tags).
 TWO_LEVEL_CFGS = [
     (dict(eviction="bfs"), 1 << 23),                                      # 512 slices: digits 8 + 1
     (dict(eviction="dfs"), 1 << 24),                                      # 8 + 2
     (dict(eviction="bfs", bits_per_tag=8, word_bytes=4, bucket_size=16), 1 << 24),  # 16-byte buckets
     (dict(eviction="bfs", bits_per_tag=8, bucket_size=4, word_bytes=4), 1 << 24),   # 4-byte buckets, 2^13-bucket slices
-    (dict(eviction="bfs", bits_per_tag=32, bucket_size=8), 1 << 23),      # too wide: LSD
+    (dict(eviction="bfs", bits_per_tag=32, bucket_size=8), 1 << 23),      # 64-bit entries in both levels
+    # 16-bit tags, 8-byte buckets, 2^25 buckets in 2^13 slices: 12 + 5 + 16 bits do
+    # not fit in 32, so the first level keeps 64-bit entries and the second packs to 32
+    (dict(eviction="bfs", bucket_size=4, word_bytes=8), 1 << 27),
     (dict(eviction="bfs", policy="addsub"), 1 << 23),
     # non-power-of-two bucket counts: 781,250 buckets in 763 slices (the last
     # partial), 393,988 in 385; some first-level partitions stay empty
