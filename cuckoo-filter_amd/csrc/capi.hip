@@ -432,6 +432,35 @@ int cf_compute_packed_tags(cf_filter* f, const void* d_keys, size_t n, void* d_p
     return e == hipSuccess ? CF_OK : hipFail(e, "compute packed tags");
 }
 
+int cf_sort_packed_tags(cf_filter* f, void* d_packed, size_t n, void* stream) {
+    int r = checkFilter(f);
+    if (r != CF_OK) return r;
+    if (n != 0 && d_packed == nullptr) return fail(CF_ERR_INVALID, "packed-tag pointer is NULL");
+    if (n >= (size_t(1) << 32)) return fail(CF_ERR_INVALID, "at most 2^32 - 1 packed tags per sort");
+    DeviceGuard g(f->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e;
+    {
+        SharedWorkspace& sw = sharedWorkspace();
+        std::lock_guard<std::mutex> lock(sw.mu);
+        e = f->ops->sortPacked(d_packed, n, f->numBuckets, s, sw.perDevice[f->device]);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    return e == hipSuccess ? CF_OK : hipFail(e, "sort packed tags");
+}
+
+int cf_insert_packed_sorted(cf_filter* f, const void* d_packed, size_t n, void* stream, size_t* occupied) {
+    int r = checkFilter(f);
+    if (r != CF_OK) return r;
+    if (n != 0 && d_packed == nullptr) return fail(CF_ERR_INVALID, "packed-tag pointer is NULL");
+    DeviceGuard g(f->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = f->ops->insertPacked(f->view(), d_packed, n, f->cfg.block_size, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hipFail(e, "insertKernelSorted");
+    return readOccupied(f, s, occupied);
+}
+
 }  // extern "C"
 
 struct cf_batch {

@@ -67,6 +67,7 @@ struct Entry {
                              cuckoo::Workspace& ws) {
         return E::insert(view(a), static_cast<const Key*>(keys), n, block, s, ws);
     }
+    static size_t insertWorkspaceBytes(size_t n, size_t nb) { return E::insertWorkspaceBytes(n, nb); }
     static hipError_t insertSorted(const cfabi::AnyView& a, const void* keys, size_t n, uint32_t block,
                                    hipStream_t s, cuckoo::Workspace& ws) {
         return E::insertSorted(view(a), static_cast<const Key*>(keys), n, block, s, ws);
@@ -85,6 +86,13 @@ struct Entry {
     }
     static hipError_t packedTags(const void* keys, size_t n, size_t nb, void* out, uint32_t block, hipStream_t s) {
         return E::packedTags(static_cast<const Key*>(keys), n, nb, static_cast<typename E::Packed*>(out), block, s);
+    }
+    static hipError_t sortPacked(void* packed, size_t n, size_t nb, hipStream_t s, cuckoo::Workspace& ws) {
+        return E::sortPacked(static_cast<typename E::Packed*>(packed), n, nb, s, ws);
+    }
+    static hipError_t insertPacked(const cfabi::AnyView& a, const void* packed, size_t n, uint32_t block,
+                                   hipStream_t s) {
+        return E::insertPacked(view(a), static_cast<const typename E::Packed*>(packed), n, block, s);
     }
 
     static hipError_t batchExecute(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t s,
@@ -116,11 +124,14 @@ struct Entry {
         o.numBuckets = &numBuckets;
         o.hostCandidates = &hostCandidates;
         o.insert = &insert;
+        o.insertWorkspaceBytes = &insertWorkspaceBytes;
         o.insertSorted = &insertSorted;
         o.contains = &contains;
         o.remove = &remove;
         o.candidates = &candidates;
         o.packedTags = &packedTags;
+        o.sortPacked = &sortPacked;
+        o.insertPacked = &insertPacked;
         o.batchExecute = &batchExecute;
         o.batchLaunch = &batchLaunch;
         o.batchResults = &batchResults;

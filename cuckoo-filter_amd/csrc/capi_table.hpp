@@ -31,6 +31,8 @@ struct Ops {
     void (*hostCandidates)(size_t nb, const void* keys, size_t n, uint64_t* i1, uint64_t* i2, uint32_t* fp);
     hipError_t (*insert)(const AnyView&, const void* keys, size_t n, uint32_t block, hipStream_t,
                          cuckoo::Workspace&);
+    // device workspace insert() takes for n keys into a table of nb buckets (0 on the direct path)
+    size_t (*insertWorkspaceBytes)(size_t n, size_t nb);
     hipError_t (*insertSorted)(const AnyView&, const void* keys, size_t n, uint32_t block, hipStream_t,
                                cuckoo::Workspace&);
     hipError_t (*contains)(const AnyView&, const void* keys, size_t n, bool* out, uint32_t block, hipStream_t);
@@ -38,6 +40,9 @@ struct Ops {
     hipError_t (*candidates)(const void* keys, size_t n, size_t nb, uint64_t* i1, uint64_t* i2, uint32_t* fp,
                              uint32_t block, hipStream_t);
     hipError_t (*packedTags)(const void* keys, size_t n, size_t nb, void* out, uint32_t block, hipStream_t);
+    // the presorted route: ascending sort of packed tags, then insertKernelSorted over them
+    hipError_t (*sortPacked)(void* packed, size_t n, size_t nb, hipStream_t, cuckoo::Workspace&);
+    hipError_t (*insertPacked)(const AnyView&, const void* packed, size_t n, uint32_t block, hipStream_t);
     // batched many-filter execution; *state is this configuration's executor (created on first use)
     hipError_t (*batchExecute)(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t,
                                uint64_t* occupiedOut);

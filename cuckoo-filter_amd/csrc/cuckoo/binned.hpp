@@ -439,16 +439,19 @@ inline int log2Floor(size_t v) {
     return b;
 }
 
-/// Insert path selection: CUCKOO_INSERT_PATH = "direct" | "binned" | "auto"
-/// (default auto: binned once the batch has at least one key per
-/// kBinnedMinBucketsPerKey buckets).
-enum class InsertPath { Auto, Direct, Binned };
+/// Insert path selection: CUCKOO_INSERT_PATH = "direct" | "binned" |
+/// "presorted" | "auto" (default auto: binned once the batch has at least one
+/// key per kBinnedMinBucketsPerKey buckets).  "presorted" sends
+/// insertManySorted down the reference's own route -- packed tags, a full
+/// ascending sort, insertKernelSorted -- at every table size.
+enum class InsertPath { Auto, Direct, Binned, Presorted };
 
 inline InsertPath insertPathFromEnv() {
     const char* e = std::getenv("CUCKOO_INSERT_PATH");
     if (e == nullptr) return InsertPath::Auto;
     if (std::strcmp(e, "direct") == 0) return InsertPath::Direct;
     if (std::strcmp(e, "binned") == 0) return InsertPath::Binned;
+    if (std::strcmp(e, "presorted") == 0) return InsertPath::Presorted;
     return InsertPath::Auto;
 }
 

@@ -25,6 +25,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <type_traits>
+#include <utility>
 
 #include "bucket_policies.hpp"
 #include "hashutil.hpp"
@@ -72,6 +73,15 @@ struct DeviceView {
     unsigned int* chainHist;           ///< per-insert eviction-chain histogram, nullptr = off
     uint32_t maxEvictions;             ///< DFS chain bound
     uint32_t evictionPolicy;           ///< 0 = BFS, 1 = DFS
+
+    DeviceView() = default;
+    /// A filter's address converts to its view, so that launches written for
+    /// the reference, which pass the filter object's address to the free
+    /// kernels (CuckooFilter.cuh:459; benchmark/cuckoo_sorted_vs_unsorted.cu:74-76),
+    /// compile unchanged and pass the view by value instead.
+    template <typename F, typename = std::enable_if_t<std::is_same_v<decltype(std::declval<const F&>().view()),
+                                                                    DeviceView>>>
+    DeviceView(const F* filter) : DeviceView(filter->view()) {}
 };
 
 namespace detail {

@@ -62,6 +62,12 @@ struct Engine {
         return hipGetLastError();
     }
 
+    /// Device workspace insert() reserves for n keys (0 on the direct path).
+    static size_t insertWorkspaceBytes(size_t n, size_t numBuckets) {
+        if (!useBinned(n, numBuckets)) return 0;
+        return Binned::workspaceBytes(std::min(n, size_t(1) << 31), numBuckets);
+    }
+
     static hipError_t contains(const View& v, const Key* keys, size_t n, bool* out, uint32_t block,
                                hipStream_t s) {
         if (n == 0) return hipSuccess;
@@ -94,14 +100,59 @@ struct Engine {
         return hipGetLastError();
     }
 
+    /// Bits that order packed tags of a table of nb buckets: the fingerprint
+    /// and the primary bucket, capped at PackedTagType's width (the
+    /// reference's u32 truncation for 8-bit tags, CuckooFilter.cuh:159).
+    static int packedBits(size_t nb) {
+        const int hi = static_cast<int>(Config::bitsPerTag) + detail::bitsFor(nb);
+        return std::min<int>(hi, static_cast<int>(sizeof(Packed) * 8));
+    }
+
+    static size_t sortWorkspaceBytes(size_t m) {
+        return ((m * sizeof(Packed) + 255) / 256) * 256 + radix::scratchBytes(m);
+    }
+
+    /// The cub::DeviceRadixSort::SortKeys step of insertManySorted
+    /// (CuckooFilter.cuh:485-498), in place: n packed tags of a table of nb
+    /// buckets in ascending order.  Every bit a packed tag of this table can
+    /// hold is sorted, so the order is the reference's full-value order.  The
+    /// LSD passes ping-pong between the workspace and `data`.  n < 2^32.
+    static hipError_t sortPacked(Packed* data, size_t n, size_t nb, hipStream_t s, Workspace& ws) {
+        if (n < 2) return hipSuccess;
+        const size_t vals = ((n * sizeof(Packed) + 255) / 256) * 256;
+        hipError_t e = ws.reserve(sortWorkspaceBytes(n));
+        if (e != hipSuccess) return e;
+        Packed* a = static_cast<Packed*>(ws.ptr);
+        void* scratch = static_cast<char*>(ws.ptr) + vals;
+        Packed* sorted = data;
+        e = radix::sortBits<Packed>(radix::ArraySource<Packed>{data}, n, 0, packedBits(nb), a, data, scratch, s,
+                                    &sorted);
+        if (e != hipSuccess) return e;
+        if (sorted != data) e = hipMemcpyAsync(data, sorted, n * sizeof(Packed), hipMemcpyDeviceToDevice, s);
+        return e;
+    }
+
+    /// insertKernelSorted over n packed tags (CuckooFilter.cuh:1209-1266), as
+    /// the reference's presorted benchmark launches it
+    /// (benchmark/cuckoo_sorted_vs_unsorted.cu:74-76).
+    static hipError_t insertPacked(const View& v, const Packed* packed, size_t n, uint32_t block, hipStream_t s) {
+        if (n == 0) return hipSuccess;
+        auto k = insertKernelSorted<Config>;
+        k<<<detail::gridFor(k, n, block), block, 0, s>>>(nullptr, packed, n, v);
+        return hipGetLastError();
+    }
+
     /// insertManySorted (CuckooFilter.cuh:475-508): keys are radix-partitioned
     /// by primary bucket (as packed into PackedTagType) and inserted slice by
-    /// slice through LDS.  Tables beyond 2^32 buckets take the packed-tag
-    /// sort + insertKernelSorted route instead.
+    /// slice through LDS.  Tables beyond 2^32 buckets, and every table when
+    /// CUCKOO_INSERT_PATH=presorted, take the reference's own route instead:
+    /// packed tags, sorted ascending over all their bits, then
+    /// insertKernelSorted.
     static hipError_t insertSorted(const View& v, const Key* keys, size_t n, uint32_t block, hipStream_t s,
                                    Workspace& ws) {
         if (n == 0) return hipSuccess;
-        if (Binned::applicable(v.numBuckets)) return Binned::template run<true>(v, keys, n, block, s, ws);
+        const bool presorted = detail::insertPathFromEnv() == detail::InsertPath::Presorted;
+        if (!presorted && Binned::applicable(v.numBuckets)) return Binned::template run<true>(v, keys, n, block, s, ws);
         constexpr size_t kChunk = size_t(1) << 31;
         for (size_t off = 0; off < n; off += kChunk) {
             const size_t m = std::min(kChunk, n - off);
@@ -111,15 +162,11 @@ struct Engine {
             Packed* a = static_cast<Packed*>(ws.ptr);
             Packed* b = reinterpret_cast<Packed*>(static_cast<char*>(ws.ptr) + vals);
             void* scratch = static_cast<char*>(ws.ptr) + 2 * vals;
-            int hi = Config::bitsPerTag + detail::bitsFor(v.numBuckets);
-            hi = std::min<int>(hi, static_cast<int>(sizeof(Packed) * 8));
             Packed* sorted = a;
-            e = radix::sortBits<Packed>(PackedTagSource<Config>{keys + off, v.numBuckets}, m, Config::bitsPerTag, hi,
-                                        a, b, scratch, s, &sorted);
+            const PackedTagSource<Config> src{keys + off, v.numBuckets};
+            e = radix::sortBits<Packed>(src, m, 0, packedBits(v.numBuckets), a, b, scratch, s, &sorted);
             if (e != hipSuccess) return e;
-            auto k = insertKernelSorted<Config>;
-            k<<<detail::gridFor(k, m, block), block, 0, s>>>(nullptr, sorted, m, v);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
+            if ((e = insertPacked(v, sorted, m, block, s)) != hipSuccess) return e;
         }
         return hipSuccess;
     }

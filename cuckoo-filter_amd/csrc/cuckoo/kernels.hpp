@@ -163,13 +163,17 @@ __global__ __launch_bounds__(kMaxBlock) void candidatesKernel(const typename Con
     }
 }
 
-/// Device memory the sorted insert needs, grown on demand and kept between calls.
+/// Device memory the sorted insert needs, grown on demand and kept between
+/// calls -- or, when `borrowed`, a fixed buffer the workspace neither grows
+/// nor frees (a request beyond it fails with hipErrorOutOfMemory).
 struct Workspace {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;
 
     hipError_t reserve(size_t want) {
         if (want <= bytes) return hipSuccess;
+        if (borrowed) return hipErrorOutOfMemory;
         if (ptr != nullptr) {
             const hipError_t e = hipFree(ptr);
             ptr = nullptr;
@@ -182,7 +186,7 @@ struct Workspace {
     }
 
     void release() {
-        if (ptr != nullptr) (void)hipFree(ptr);
+        if (ptr != nullptr && !borrowed) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
     }
@@ -190,25 +194,43 @@ struct Workspace {
 
 namespace detail {
 
-/// Grid for a grid-stride kernel: enough blocks for n keys, capped at
-/// `wavesPerSimd` resident waves on every SIMD of the device.
-template <typename Kernel>
-inline uint32_t gridFor(Kernel kernel, size_t n, uint32_t block) {
-    const size_t need = (n + block - 1) / block;
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    static thread_local int cachedDev = -1;
-    static thread_local int cus = 0;
-    if (cachedDev != dev) {
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        cachedDev = dev;
+/// Resident workgroups of `kernel` at `block` threads on device `dev`: the
+/// occupancy query runs once per (kernel, block, device) and thread, not per
+/// launch.
+inline uint32_t residentBlocks(const void* kernel, uint32_t block, int dev) {
+    struct Entry {
+        const void* kernel;
+        uint32_t block;
+        int dev;
+        uint32_t resident;
+    };
+    constexpr int kEntries = 64;
+    static thread_local Entry cache[kEntries] = {};
+    static thread_local int next = 0;
+    for (const Entry& e : cache) {
+        if (e.kernel == kernel && e.block == block && e.dev == dev) return e.resident;
     }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
     int perCu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, static_cast<int>(block), 0) != hipSuccess ||
         perCu <= 0) {
         perCu = 1;
     }
-    const size_t cap = static_cast<size_t>(perCu) * static_cast<size_t>(std::max(cus, 1));
+    const uint32_t resident = static_cast<uint32_t>(perCu) * static_cast<uint32_t>(cus);
+    cache[next] = Entry{kernel, block, dev, resident};
+    next = (next + 1) % kEntries;
+    return resident;
+}
+
+/// Grid for a grid-stride kernel: enough blocks for n keys, capped at the
+/// number of workgroups the device keeps resident.
+template <typename Kernel>
+inline uint32_t gridFor(Kernel kernel, size_t n, uint32_t block) {
+    const size_t need = (n + block - 1) / block;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const size_t cap = residentBlocks(reinterpret_cast<const void*>(kernel), block, dev);
     return static_cast<uint32_t>(std::max<size_t>(1, std::min(need, cap)));
 }
 

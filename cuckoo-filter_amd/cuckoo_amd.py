@@ -121,6 +121,8 @@ def _load():
         "cf_import_table": (i32, [P, P, sz, sz]),
         "cf_compute_candidates": (i32, [P, P, sz, P, P, P, P]),
         "cf_compute_packed_tags": (i32, [P, P, sz, P, P]),
+        "cf_sort_packed_tags": (i32, [P, P, sz, P]),
+        "cf_insert_packed_sorted": (i32, [P, P, sz, P, C.POINTER(sz)]),
         "cf_copy_filter": (i32, [P, P, P]),
         "cf_batch_create": (i32, [C.POINTER(P)]),
         "cf_batch_destroy": (i32, [P]),
@@ -397,6 +399,30 @@ class CuckooFilter:
         _check(lib.cf_compute_packed_tags(self._h, C.c_void_p(k.data_ptr()), k.numel(),
                                           C.c_void_p(out.data_ptr()), _stream_ptr(stream)))
         return out
+
+    def sortPackedTags(self, packed, stream=None):
+        """In-place ascending sort of packed tags (the cub::DeviceRadixSort
+        step of insertManySorted, CuckooFilter.cuh:485-498)."""
+        self._packed(packed)
+        _check(lib.cf_sort_packed_tags(self._h, C.c_void_p(packed.data_ptr()), packed.numel(),
+                                       _stream_ptr(stream)))
+        return packed
+
+    def insertPackedSorted(self, packed, stream=None) -> int:
+        """insertKernelSorted over packed tags (CuckooFilter.cuh:1209-1266);
+        returns the total occupancy afterwards."""
+        self._packed(packed)
+        occ = C.c_size_t()
+        _check(lib.cf_insert_packed_sorted(self._h, C.c_void_p(packed.data_ptr()), packed.numel(),
+                                           _stream_ptr(stream), C.byref(occ)))
+        return occ.value
+
+    def _packed(self, packed):
+        torch = _torch()
+        want = 4 if self.config.bits_per_tag <= 8 else 8
+        if (not isinstance(packed, torch.Tensor) or not packed.is_cuda or packed.element_size() != want
+                or packed.dtype.is_floating_point or not packed.is_contiguous()):
+            raise TypeError(f"packed tags must be a contiguous device tensor of {8 * want}-bit integers")
 
 
 class Batch:

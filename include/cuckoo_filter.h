@@ -146,6 +146,22 @@ int cf_compute_candidates(cf_filter* f, const void* d_keys, size_t n, uint64_t* 
  * PackedTagType values (uint32_t when bits_per_tag <= 8, else uint64_t). */
 int cf_compute_packed_tags(cf_filter* f, const void* d_keys, size_t n, void* d_packed, void* stream);
 
+/* The cub::DeviceRadixSort::SortKeys step of insertManySorted
+ * (CuckooFilter.cuh:485-498; the reference's presorted benchmark calls it
+ * directly, benchmark/cuckoo_sorted_vs_unsorted.cu:57-69): sorts n packed
+ * tags of this filter's table in place, ascending.  Every bit such a value
+ * can hold (bits_per_tag + ceil(log2(num_buckets)), capped at the
+ * PackedTagType width) is ordered, so for values cf_compute_packed_tags
+ * produced the result is the full ascending order.  n < 2^32. */
+int cf_sort_packed_tags(cf_filter* f, void* d_packed, size_t n, void* stream);
+
+/* insertKernelSorted (CuckooFilter.cuh:1209-1266) over n PackedTagType
+ * values, as the reference's InsertPresorted benchmark launches it
+ * (benchmark/cuckoo_sorted_vs_unsorted.cu:74-76); *occupied receives the
+ * total occupancy afterwards.  The values need not be sorted; sorted ones
+ * give neighbouring threads neighbouring buckets. */
+int cf_insert_packed_sorted(cf_filter* f, const void* d_packed, size_t n, void* stream, size_t* occupied);
+
 /* ------------------------------------------------------------------------ */
 /* Batched many-filter execution (no reference counterpart: replaces a loop   */
 /* of the calls above, each of which launches, synchronises and reads the    */

@@ -246,9 +246,14 @@ class Filter:
 
     def table_bytes(self) -> np.ndarray:
         """Copy of the raw table image (little-endian words)."""
+        return self.table_view().copy()
+
+    def table_view(self) -> np.ndarray:
+        """The raw table image in place (writes change the filter's table;
+        valid while the filter lives)."""
         p = lib().cfo_table(self._h)
         n = self.size_bytes
-        return np.ctypeslib.as_array((C.c_uint8 * n).from_address(p)).copy()
+        return np.ctypeslib.as_array((C.c_uint8 * n).from_address(p))
 
     def load_table(self, image: np.ndarray, occupied: int):
         img = np.ascontiguousarray(image, dtype=np.uint8)

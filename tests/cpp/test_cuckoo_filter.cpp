@@ -7,6 +7,7 @@
 //
 // Build: hipcc --offload-arch=gfx950 -std=c++20 -O2 -I cuckoo-filter_amd/csrc
 // Run:   ./test_cuckoo_filter   (exit status 0 iff every check holds)
+#include <hipcub/hipcub.hpp>
 #include <thrust/copy.h>
 #include <thrust/device_vector.h>
 
@@ -15,6 +16,7 @@
 #include <cstdlib>
 #include <functional>
 #include <limits>
+#include <memory>
 #include <random>
 #include <string>
 #include <vector>
@@ -217,6 +219,42 @@ void hostHashMatchesDevice(std::mt19937& rng) {
     check(same == keys.size(), "host getCandidateBuckets == device computePackedTagsKernel");
 }
 
+void presortedInsert(std::mt19937& rng) {
+    // The reference's InsertPresorted benchmark flow
+    // (benchmark/cuckoo_sorted_vs_unsorted.cu:48-86), written as it is there:
+    // computePackedTagsKernel, a device radix sort of the packed tags, then
+    // insertKernelSorted launched with the filter's address.
+    using PackedTagType = Filter::PackedTagType;
+    const size_t n = static_cast<size_t>((1u << 20) * 0.95);
+    auto filter = std::make_unique<Filter>(n);
+    const auto keys = randomKeys(rng, n);
+    thrust::device_vector<uint32_t> d_keys(keys.begin(), keys.end());
+    PackedTagType* d_packedTags = nullptr;
+    HIP_CALL(hipMalloc(&d_packedTags, n * sizeof(PackedTagType)));
+    const size_t numBlocks = SDIV(n, Config::blockSize);
+    computePackedTagsKernel<Config><<<numBlocks, Config::blockSize>>>(thrust::raw_pointer_cast(d_keys.data()),
+                                                                      d_packedTags, n, filter->numBuckets);
+    void* d_tempStorage = nullptr;
+    size_t tempStorageBytes = 0;
+    HIP_CALL(hipcub::DeviceRadixSort::SortKeys(d_tempStorage, tempStorageBytes, d_packedTags, d_packedTags, n));
+    HIP_CALL(hipMalloc(&d_tempStorage, tempStorageBytes));
+    HIP_CALL(hipcub::DeviceRadixSort::SortKeys(d_tempStorage, tempStorageBytes, d_packedTags, d_packedTags, n));
+    HIP_CALL(hipFree(d_tempStorage));
+    HIP_CALL(hipDeviceSynchronize());
+    std::vector<PackedTagType> h(n);
+    HIP_CALL(hipMemcpy(h.data(), d_packedTags, n * sizeof(PackedTagType), hipMemcpyDeviceToHost));
+    check(std::is_sorted(h.begin(), h.end()), "packed tags sorted");
+    filter->clear();
+    insertKernelSorted<Config><<<numBlocks, Config::blockSize>>>(thrust::raw_pointer_cast(d_keys.data()),
+                                                                 d_packedTags, n, filter.get());
+    HIP_CALL(hipDeviceSynchronize());
+    const size_t inserted = filter->occupiedSlots();
+    check(inserted > n * 0.99, "presorted insert places at least 99% of the keys");
+    check(filter->countOccupiedSlots() == inserted, "counter equals non-empty slots");
+    check(countTrue(lookup(*filter, keys)) >= inserted, "presorted-inserted keys found");
+    HIP_CALL(hipFree(d_packedTags));
+}
+
 }  // namespace
 
 int main() {
@@ -235,6 +273,7 @@ int main() {
         {"DeleteDuplicates", deleteDuplicates},
         {"SortedInsertAndCounters", sortedInsertAndCounters},
         {"HostHashMatchesDevice", hostHashMatchesDevice},
+        {"PresortedInsert", presortedInsert},
     };
     for (const auto& [name, fn] : tests) {
         std::mt19937 rng(42);

@@ -82,3 +82,80 @@ def test_config_at_98_percent(cf, oracle_mod, kw, eviction):
         assert f.deleteMany(dpos, out=out) == 0
         assert out.all().item()
     f.close()
+
+
+# ---------------------------------------------------------------------------
+# The autotuner's sweep points against the sequential oracle.
+# ---------------------------------------------------------------------------
+#
+# The parameter sweep (uint64 keys, XOR policy; tag bits x bucket size x word
+# size x eviction policy at 98% load) at 2^20 slots: each point's failure
+# count, false-positive count and eviction-chain histogram against the
+# sequential oracle filling the same keys in key order.
+#
+# The GPU insert is concurrent, so it is a different order of the same
+# algorithm, and its eviction work differs from the key-order run by what the
+# order does: the binned insert first tries every key's primary bucket, then
+# the secondaries, then evicts.  The bounds below are therefore two-sample
+# bounds with an allowance for that difference in order, not an identity
+# (which the one-key-per-call tests in test_gpu_parity.py check, histogram
+# included):
+#   * failures: |gpu - oracle| <= 4 sqrt(gpu + oracle) + 0.25 max(gpu, oracle) + 3;
+#   * false positives on 2^18 absent keys: two-sample binomial, 4.5 sigma;
+#   * histogram over the bins 1, 2, 3-4, 5-8, 9-16, 17-64, 65-max: keys that
+#     evicted at all within 20% (plus 5 sigma) of the oracle's, and the mean
+#     chain within 25% (plus 0.3) of the oracle's.
+
+SWEEP = []
+for bits, bucket, wb in itertools.product((8, 16, 32), (4, 8, 16, 32), (4, 8)):
+    tb = {8: 1, 16: 2, 32: 4}[bits]
+    if tb > wb or bucket % (wb // tb):
+        continue
+    SWEEP.append(dict(bits_per_tag=bits, bucket_size=bucket, word_bytes=wb))
+assert len(SWEEP) == 23
+
+CHAIN_BINS = [(1, 1), (2, 2), (3, 4), (5, 8), (9, 16), (17, 64), (65, None)]
+
+
+def _coarse(hist):
+    """Counts of successful evicting inserts per chain-length bin (hist bin c
+    counts inserts that made c swaps; the last bin counts failures)."""
+    top = hist.size - 2
+    return np.array([hist[lo:(top if hi is None else hi) + 1].sum() for lo, hi in CHAIN_BINS], np.int64)
+
+
+@pytest.mark.parametrize("eviction", ["bfs", "dfs"])
+@pytest.mark.parametrize("kw", SWEEP, ids=lambda d: "t{bits_per_tag}-b{bucket_size}-w{word_bytes}".format(**d))
+def test_sweep_point_against_oracle(cf, oracle_mod, kw, eviction):
+    O = oracle_mod
+    c = cf.make_config(key_bytes=8, eviction=eviction, **kw)
+    oc = O.make_config(**c.as_dict())
+    f = cf.CuckooFilter(CAP, config=c)
+    f.setEvictionHistogram(True)
+    n = int(f.capacity() * ALPHA)
+    pos, neg = _key_sets(O, 8, n)
+    occ = f.insertMany(cf.keys_to_device(pos))
+    gh = f.evictionHistogram()
+    of = O.Filter(oc, CAP)
+    occ_o, ok_o, chain = of.insert_many(pos, want_chain=True)
+    oh = np.zeros_like(gh)
+    np.add.at(oh, chain[ok_o & (chain > 0)].astype(np.int64), 1)
+    oh[-1] = int((~ok_o).sum())
+    fail_g, fail_o = n - occ, n - occ_o
+    assert int(gh[-1]) == fail_g and int(gh[1:].sum()) <= n
+    cg, co = _coarse(gh), _coarse(oh)
+    ev_g, ev_o = int(cg.sum()), int(co.sum())
+    lens = np.arange(gh.size - 1)
+    mean_g = float((gh[1:-1] * lens[1:]).sum()) / max(ev_g, 1)
+    mean_o = float((oh[1:-1] * lens[1:]).sum()) / max(ev_o, 1)
+    fp_g = int(f.containsMany(cf.keys_to_device(neg)).sum().item())
+    fp_o = int(of.contains_many(neg).sum())
+    print(f"{kw} {eviction}: failed {fail_g}/{fail_o}, evicting {ev_g}/{ev_o}, mean chain "
+          f"{mean_g:.2f}/{mean_o:.2f}, bins {cg.tolist()}/{co.tolist()}, false positives {fp_g}/{fp_o}",
+          flush=True)
+    assert abs(fail_g - fail_o) <= 4 * np.sqrt(fail_g + fail_o) + 0.25 * max(fail_g, fail_o) + 3
+    pooled = (fp_g + fp_o) / (2 * neg.size)
+    assert abs(fp_g - fp_o) <= 4.5 * np.sqrt(2 * neg.size * pooled * (1 - pooled)) + 3
+    assert abs(ev_g - ev_o) <= 0.2 * ev_o + 5 * np.sqrt(ev_g + ev_o) + 3, (ev_g, ev_o)
+    assert abs(mean_g - mean_o) <= 0.25 * mean_o + 0.3, (mean_g, mean_o)
+    f.close()

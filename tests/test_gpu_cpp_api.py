@@ -15,4 +15,4 @@ def test_cpp_template_api_behaviour(device):
     r = subprocess.run([EXE], capture_output=True, text=True, timeout=240)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("[  OK  ]") == 14
+    assert r.stdout.count("[  OK  ]") == 15

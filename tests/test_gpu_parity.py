@@ -156,18 +156,26 @@ def test_single_key_calls_bit_exact(cf, oracle_mod, kw, path, monkeypatch):
     cap = 64 * c.bucket_size
     of = O.Filter(oc, cap)
     f = cf.CuckooFilter(cap, config=c)
+    f.setEvictionHistogram(True)
     n = int(of.capacity() * 1.06)
     keys = O.gen_keys(31, n, c.key_bytes)
     dk = _dev_keys(cf, keys)
     fails = 0
+    want_hist = np.zeros(c.max_evictions + 2, np.uint64)
     for i in range(n):
         occ_gpu = f.insertMany(dk[i:i + 1])
-        occ_cpu, ok = of.insert_many(keys[i:i + 1])
+        occ_cpu, ok, chain = of.insert_many(keys[i:i + 1], want_chain=True)
         fails += int(not ok[0])
+        if not ok[0]:
+            want_hist[-1] += 1
+        elif chain[0]:
+            want_hist[chain[0]] += 1
         assert occ_gpu == occ_cpu, (i, occ_gpu, occ_cpu)
     np.testing.assert_array_equal(f.exportTable(), of.table_bytes())
     assert fails > 0  # exhausted chains were exercised
     assert f.evictionCount() == of.evictions()
+    # the eviction-chain histogram, bin for bin
+    np.testing.assert_array_equal(f.evictionHistogram(), want_hist)
     # ordered single-key deletes of every other key
     out = __import__("torch").zeros(1, dtype=__import__("torch").bool, device=dk.device)
     for i in range(0, n, 2):
@@ -220,11 +228,14 @@ BATCH_CFGS = [
 ]
 
 
-@pytest.mark.parametrize("mode", ["direct", "binned", "sorted"])
+@pytest.mark.parametrize("mode", ["direct", "binned", "sorted", "presorted"])
 @pytest.mark.parametrize("kw", BATCH_CFGS, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
 def test_batch_insert_invariants_and_binomial_parity(cf, oracle_mod, kw, mode, monkeypatch):
-    sorted_insert = mode == "sorted"
-    if not sorted_insert:
+    """mode: insertMany's direct and binned paths, insertManySorted's binned
+    path, and insertManySorted down the reference's own presorted route
+    (packed tags, full sort, insertKernelSorted)."""
+    sorted_insert = mode in ("sorted", "presorted")
+    if mode != "sorted":
         monkeypatch.setenv("CUCKOO_INSERT_PATH", mode)
     O = oracle_mod
     c = cf.make_config(**kw)

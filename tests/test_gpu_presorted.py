@@ -1,0 +1,134 @@
+"""The reference's presorted insert route on the GPU, through the C ABI.
+
+insertManySorted in the reference computes packed tags
+(computePackedTagsKernel, include/CuckooFilter.cuh:1185-1207), sorts them
+with cub::DeviceRadixSort (:485-498) and inserts them with insertKernelSorted
+(:1209-1266); its InsertPresorted benchmark launches those three steps itself
+(benchmark/cuckoo_sorted_vs_unsorted.cu:48-86).  Here the same steps are
+cf_compute_packed_tags, cf_sort_packed_tags and cf_insert_packed_sorted, and
+insertManySorted takes them end to end under CUCKOO_INSERT_PATH=presorted.
+
+  * the sort is bit-exact against numpy's sort of the oracle's packed tags;
+  * one-key-per-call inserts down the route give the sequential oracle's
+    table image after every call, eviction chains and exhausted chains
+    included;
+  * a whole batch keeps the invariants of the concurrent insert (tests in
+    test_gpu_parity.py run the "presorted" mode over nine configurations).
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_table_count
+
+pytestmark = pytest.mark.gpu
+
+PK_DT = {True: np.uint32, False: np.uint64}  # PackedTagType: u32 when bits_per_tag <= 8
+
+
+def _packed_host(cf, t, bits):
+    return cf.to_numpy_u(t, PK_DT[bits <= 8]).astype(np.uint64)
+
+
+# (config, capacity): the LSD passes over bits_per_tag + log2(numBuckets)
+# bits number 3 (8-bit tags, 2^16 buckets: 24 bits), 4 (16-bit, 2^16: 32), 5
+# (16-bit, 2^24: 40) and 6 (32-bit, 2^16: 48), so the result lands in the
+# workspace (odd counts, then copied back) and in place (even counts); 8-bit
+# tags at 2^26 buckets keep only the 24 bucket bits a u32 packed tag holds
+SORT_CASES = [
+    (dict(bits_per_tag=8, word_bytes=4), 1 << 20),
+    (dict(bits_per_tag=16), 1 << 20),
+    (dict(bits_per_tag=16), 1 << 28),
+    (dict(bits_per_tag=32), 1 << 20),
+    (dict(bits_per_tag=8, word_bytes=4), 1 << 30),
+    (dict(bits_per_tag=16, policy="addsub"), 12_500_000),
+]
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 4095, 4097, 100_003, 1 << 20])
+@pytest.mark.parametrize("kw,cap", SORT_CASES, ids=lambda x: str(x))
+def test_sort_packed_tags_is_the_full_ascending_order(cf, oracle_mod, kw, cap, n):
+    O = oracle_mod
+    c = cf.make_config(**kw)
+    f = cf.CuckooFilter(cap, config=c)
+    nb = f.getNumBuckets()
+    keys = O.gen_keys(7 + n, max(n, 1), c.key_bytes)[:n]
+    want = np.sort(O.packed_tags(O.make_config(**c.as_dict()), nb, keys))
+    import torch
+    dk = cf.keys_to_device(keys) if n else torch.empty(0, dtype=torch.int64, device="cuda")
+    packed = f.computePackedTags(dk)
+    np.testing.assert_array_equal(np.sort(_packed_host(cf, packed, c.bits_per_tag)), want)
+    f.sortPackedTags(packed)
+    np.testing.assert_array_equal(_packed_host(cf, packed, c.bits_per_tag), want)
+    f.close()
+
+
+SEQ_CFGS = [
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, eviction="bfs", max_evictions=40),
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, eviction="dfs", max_evictions=40),
+    dict(bits_per_tag=8, bucket_size=16, word_bytes=4, eviction="dfs", max_evictions=30),
+    dict(bits_per_tag=32, bucket_size=8, word_bytes=8, eviction="bfs", max_evictions=30),
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, eviction="bfs", policy="addsub", max_evictions=30),
+    dict(bits_per_tag=16, bucket_size=16, word_bytes=8, eviction="dfs", policy="offset", max_evictions=30),
+    dict(bits_per_tag=8, bucket_size=8, word_bytes=8, eviction="bfs", policy="offset", max_evictions=30),
+]
+
+
+@pytest.mark.parametrize("entry", ["insertManySorted", "insertPackedSorted"])
+@pytest.mark.parametrize("kw", SEQ_CFGS, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_presorted_single_key_calls_bit_exact(cf, oracle_mod, kw, entry, monkeypatch):
+    """insertKernelSorted, one key per call, against the sequential oracle:
+    occupancy after every call, then the table image and eviction count."""
+    monkeypatch.setenv("CUCKOO_INSERT_PATH", "presorted")
+    O = oracle_mod
+    c = cf.make_config(**kw)
+    oc = O.make_config(**c.as_dict())
+    cap = 64 * c.bucket_size
+    of = O.Filter(oc, cap)
+    f = cf.CuckooFilter(cap, config=c)
+    n = int(of.capacity() * 1.06)
+    keys = O.gen_keys(53, n, c.key_bytes)
+    dk = cf.keys_to_device(keys)
+    packed = f.computePackedTags(dk) if entry == "insertPackedSorted" else None
+    fails = 0
+    for i in range(n):
+        if packed is None:
+            occ_gpu = f.insertManySorted(dk[i:i + 1])
+        else:
+            occ_gpu = f.insertPackedSorted(packed[i:i + 1])
+        before = of.occupied()
+        occ_cpu = of.insert_many_sorted(keys[i:i + 1])
+        fails += int(occ_cpu == before)
+        assert occ_gpu == occ_cpu, (i, occ_gpu, occ_cpu)
+    np.testing.assert_array_equal(f.exportTable(), of.table_bytes())
+    assert fails > 0  # exhausted chains were exercised
+    assert f.evictionCount() == of.evictions()
+    f.close()
+
+
+@pytest.mark.parametrize("kw", [dict(eviction="bfs"), dict(eviction="dfs"), dict(bits_per_tag=8, word_bytes=4),
+                                dict(eviction="bfs", policy="offset")],
+                         ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_presorted_flow_batch(cf, oracle_mod, kw):
+    """The InsertPresorted benchmark's three steps on a 2^20-slot filter at
+    95% load: sorted packed tags equal numpy's sort of the oracle's, then
+    insertKernelSorted keeps the table count and loses no key."""
+    import torch
+
+    O = oracle_mod
+    c = cf.make_config(**kw)
+    oc = O.make_config(**c.as_dict())
+    f = cf.CuckooFilter(1 << 20, config=c)
+    nb = f.getNumBuckets()
+    n = int(f.capacity() * 0.95)
+    keys = O.gen_keys(61, n, c.key_bytes)
+    dk = cf.keys_to_device(keys)
+    packed = f.sortPackedTags(f.computePackedTags(dk))
+    np.testing.assert_array_equal(_packed_host(cf, packed, c.bits_per_tag), np.sort(O.packed_tags(oc, nb, keys)))
+    occ = f.insertPackedSorted(packed)
+    assert occ >= n - 2, (occ, n)
+    assert_table_count(f, occ, n)
+    if occ == n:
+        assert f.containsMany(dk).all().item()
+        out = torch.zeros(n // 2, dtype=torch.bool, device=dk.device)
+        assert f.deleteMany(dk[: n // 2], out=out) == n - n // 2 and out.all().item()
+    f.close()
