@@ -136,6 +136,8 @@ def measure(kw, slots, load, seed=1, device="cuda", primary_sample=1 << 16):
             "mean_chain": round(float((hist[1:-1] * chains[1:]).sum() / max(placed_by_eviction, 1)), 3),
             "max_chain": int(chains[1:][hist[1:-1] > 0].max()) if placed_by_eviction else 0,
             "exhausted": int(hist[-1]),
+            # eviction-chain histogram: swaps made -> successful inserts (0 swaps: keys - the rest)
+            "chain_histogram": {str(c): int(hist[c]) for c in range(1, hist.size - 1) if hist[c]},
             "primary_share": round(share, 4),
             # u32 words, two per bucket: the reference's 128-bit lookup reads past
             # the bucket here (CuckooFilter.cuh:272-277, 332-342); this build's

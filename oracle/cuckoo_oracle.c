@@ -454,8 +454,9 @@ static int insert_dfs(cfo_filter* f, uint32_t fp, uint64_t start, uint32_t* chai
     return 0;
 }
 
-/* insertWithEvictionBFS, CuckooFilter.cuh:967-1018 */
-static int insert_bfs(cfo_filter* f, uint32_t fp, uint64_t start, uint32_t* chain) {
+/* The candidate loop of insertWithEvictionBFS, CuckooFilter.cuh:967-1015
+ * (everything before its DFS fallback). */
+static int bfs_candidates(cfo_filter* f, uint32_t fp, uint64_t start, uint32_t* chain) {
     uint32_t half = f->cfg.bucket_size / 2;
     uint32_t ncand = half < 8 ? half : 8;
     for (uint32_t i = 0; i < ncand; ++i) {
@@ -483,7 +484,12 @@ static int insert_bfs(cfo_filter* f, uint32_t fp, uint64_t start, uint32_t* chai
             try_remove(f, alt, cand); /* :1011-1012 (removes cand, as written) */
         }
     }
-    return insert_dfs(f, fp, start, chain);
+    return 0;
+}
+
+/* insertWithEvictionBFS, CuckooFilter.cuh:967-1018 */
+static int insert_bfs(cfo_filter* f, uint32_t fp, uint64_t start, uint32_t* chain) {
+    return bfs_candidates(f, fp, start, chain) || insert_dfs(f, fp, start, chain);
 }
 
 static int evict(cfo_filter* f, uint32_t fp, uint64_t start, uint32_t* chain) {
@@ -625,6 +631,55 @@ uint64_t cfo_insert_many_sorted(cfo_filter* f, const void* keys, size_t n, uint3
         s += (uint64_t)insert_from(f, primary, fp, chain ? &chain[i] : NULL);
     }
     free(packed);
+    f->occupied += s;
+    return f->occupied;
+}
+
+/* The reference's insert (CuckooFilter.cuh:1029-1080), every key's steps in
+ * the same order as one key alone, but scheduled by step over the batch:
+ * every key's primary-bucket attempt first, then the secondary attempts of
+ * the keys still unplaced, then (BFS policy) their BFS candidate loops, then
+ * DFS chains for the rest.  Each phase runs in key order.  This is the
+ * schedule of the bucket-binned GPU insert (a legal interleaving of the
+ * reference's concurrent insertKernel), run sequentially. */
+uint64_t cfo_insert_many_phased(cfo_filter* f, const void* keys, size_t n, uint8_t* ok, uint32_t* chain) {
+    const uint8_t* k = (const uint8_t*)keys;
+    uint64_t* start = (uint64_t*)malloc((n ? n : 1) * sizeof(uint64_t));
+    uint32_t* sfp = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint8_t* done = (uint8_t*)calloc(n ? n : 1, 1);
+    uint64_t s = 0;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t i1, i2;
+        uint32_t fp;
+        if (chain) chain[i] = 0;
+        candidates_one(&f->cfg, f->nb, k + i * f->cfg.key_bytes, &i1, &i2, &fp);
+        start[i] = i1;
+        sfp[i] = fp;
+        done[i] = (uint8_t)try_insert(f, i1, fp);
+    }
+    for (size_t i = 0; i < n; ++i) {
+        if (done[i]) continue;
+        uint32_t fp = sfp[i], fp2;
+        uint64_t i2 = cfo_alt_bucket(&f->cfg, f->nb, start[i], fp, &fp2);
+        done[i] = (uint8_t)try_insert(f, i2, fp2);
+        if (!done[i] && (fp & 1u) != 0) { /* :1065-1066: eviction starts at i2 for odd fp */
+            start[i] = i2;
+            sfp[i] = fp2;
+        }
+    }
+    if (f->cfg.eviction == CFO_EVICT_BFS) {
+        for (size_t i = 0; i < n; ++i)
+            if (!done[i]) done[i] = (uint8_t)bfs_candidates(f, sfp[i], start[i], chain ? &chain[i] : NULL);
+    }
+    for (size_t i = 0; i < n; ++i)
+        if (!done[i]) done[i] = (uint8_t)insert_dfs(f, sfp[i], start[i], chain ? &chain[i] : NULL);
+    for (size_t i = 0; i < n; ++i) {
+        s += done[i];
+        if (ok) ok[i] = done[i];
+    }
+    free(start);
+    free(sfp);
+    free(done);
     f->occupied += s;
     return f->occupied;
 }

@@ -101,6 +101,9 @@ void cfo_set_occupied(cfo_filter* f, uint64_t occupied);
 uint64_t cfo_insert_many(cfo_filter* f, const void* keys, size_t n, uint8_t* ok,
                          uint32_t* chain, int threads);
 uint64_t cfo_insert_many_sorted(cfo_filter* f, const void* keys, size_t n, uint32_t* chain);
+/* Sequential run of the insert scheduled by step: all primary attempts, then
+ * all secondary attempts, then (BFS) all BFS candidate loops, then DFS. */
+uint64_t cfo_insert_many_phased(cfo_filter* f, const void* keys, size_t n, uint8_t* ok, uint32_t* chain);
 void cfo_contains_many(cfo_filter* f, const void* keys, size_t n, uint8_t* out, int threads);
 uint64_t cfo_delete_many(cfo_filter* f, const void* keys, size_t n, uint8_t* ok, int threads);
 

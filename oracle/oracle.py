@@ -80,6 +80,7 @@ def lib():
             "cfo_set_occupied": (None, [P, u64]),
             "cfo_insert_many": (u64, [P, P, sz, P, P, C.c_int]),
             "cfo_insert_many_sorted": (u64, [P, P, sz, P]),
+            "cfo_insert_many_phased": (u64, [P, P, sz, P, P]),
             "cfo_contains_many": (None, [P, P, sz, P, C.c_int]),
             "cfo_delete_many": (u64, [P, P, sz, P, C.c_int]),
             "cfo_bucket_contains": (C.c_int, [P, u64, u32]),
@@ -213,6 +214,16 @@ class Filter:
         chain = np.zeros(keys.size, np.uint32) if want_chain else None
         occ = lib().cfo_insert_many_sorted(self._h, _ptr(keys), keys.size, _ptr(chain))
         return (int(occ), chain) if want_chain else int(occ)
+
+    def insert_many_phased(self, keys):
+        """The insert scheduled by step over the batch (all primary attempts,
+        then secondaries, then BFS loops, then DFS), sequentially; returns
+        (occupied, ok, chain)."""
+        keys = self._k(keys)
+        ok = np.zeros(keys.size, np.uint8)
+        chain = np.zeros(keys.size, np.uint32)
+        occ = lib().cfo_insert_many_phased(self._h, _ptr(keys), keys.size, _ptr(ok), _ptr(chain))
+        return int(occ), ok.astype(bool), chain
 
     def contains_many(self, keys, threads=1) -> np.ndarray:
         keys = self._k(keys)

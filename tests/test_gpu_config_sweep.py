@@ -90,21 +90,33 @@ def test_config_at_98_percent(cf, oracle_mod, kw, eviction):
 #
 # The parameter sweep (uint64 keys, XOR policy; tag bits x bucket size x word
 # size x eviction policy at 98% load) at 2^20 slots: each point's failure
-# count, false-positive count and eviction-chain histogram against the
-# sequential oracle filling the same keys in key order.
+# count, false positives and eviction-chain histogram against the oracle
+# filling the same keys.
 #
-# The GPU insert is concurrent, so it is a different order of the same
-# algorithm, and its eviction work differs from the key-order run by what the
-# order does: the binned insert first tries every key's primary bucket, then
-# the secondaries, then evicts.  The bounds below are therefore two-sample
-# bounds with an allowance for that difference in order, not an identity
-# (which the one-key-per-call tests in test_gpu_parity.py check, histogram
-# included):
-#   * failures: |gpu - oracle| <= 4 sqrt(gpu + oracle) + 0.25 max(gpu, oracle) + 3;
+# A batch insert is concurrent, and how many keys end up evicting, and how
+# far, depends on the order in which the keys' steps run.  So each GPU path
+# is compared with a sequential oracle run of its own schedule:
+#   * the bucket-binned path (what insertMany takes for batches this dense)
+#     runs every key's primary attempt, then the secondary attempts, then the
+#     BFS loops, then DFS: the oracle's insert_many_phased runs that schedule
+#     in key order;
+#   * the direct path (one thread per key, the reference's insertKernel) is
+#     compared with the key-order oracle; about 10^5 keys are in flight at
+#     once, so it evicts some 8% fewer keys than a strictly sequential run
+#     (measured over all 46 points: 6-9% fewer, mean chain 3-22% longer).
+# Bounds:
 #   * false positives on 2^18 absent keys: two-sample binomial, 4.5 sigma;
-#   * histogram over the bins 1, 2, 3-4, 5-8, 9-16, 17-64, 65-max: keys that
-#     evicted at all within 20% (plus 5 sigma) of the oracle's, and the mean
-#     chain within 25% (plus 0.3) of the oracle's.
+#   * failures: |gpu - oracle| <= 4 sqrt(gpu + oracle) + 0.25 max + 3;
+#   * keys that evicted: binned 4.5 sigma + 1% of the phased run's (measured:
+#     within 0.4%); direct within 12% of the key-order run's;
+#   * mean chain length of the evicting keys within 35% (+0.3) of the oracle's
+#     (measured: binned within 7% of the phased run, except with 4-slot
+#     buckets, where 2.4% of the keys exhaust their chains and the concurrent
+#     eviction phase gives 20% more failures and 25% shorter mean chains).
+# Chain steps inside the eviction phase also run concurrently, so the shape
+# of the histogram is not an identity either; it is printed for every point,
+# and the one-key-per-call tests in test_gpu_parity.py compare histograms bin
+# for bin where the order is fixed.
 
 SWEEP = []
 for bits, bucket, wb in itertools.product((8, 16, 32), (4, 8, 16, 32), (4, 8)):
@@ -124,10 +136,19 @@ def _coarse(hist):
     return np.array([hist[lo:(top if hi is None else hi) + 1].sum() for lo, hi in CHAIN_BINS], np.int64)
 
 
+def _oracle_hist(ok, chain, bins):
+    h = np.zeros(bins, np.int64)
+    np.add.at(h, chain[ok & (chain > 0)].astype(np.int64), 1)
+    h[-1] = int((~ok).sum())
+    return h
+
+
+@pytest.mark.parametrize("path", ["binned", "direct"])
 @pytest.mark.parametrize("eviction", ["bfs", "dfs"])
 @pytest.mark.parametrize("kw", SWEEP, ids=lambda d: "t{bits_per_tag}-b{bucket_size}-w{word_bytes}".format(**d))
-def test_sweep_point_against_oracle(cf, oracle_mod, kw, eviction):
+def test_sweep_point_against_oracle(cf, oracle_mod, kw, eviction, path, monkeypatch):
     O = oracle_mod
+    monkeypatch.setenv("CUCKOO_INSERT_PATH", path)
     c = cf.make_config(key_bytes=8, eviction=eviction, **kw)
     oc = O.make_config(**c.as_dict())
     f = cf.CuckooFilter(CAP, config=c)
@@ -135,12 +156,13 @@ def test_sweep_point_against_oracle(cf, oracle_mod, kw, eviction):
     n = int(f.capacity() * ALPHA)
     pos, neg = _key_sets(O, 8, n)
     occ = f.insertMany(cf.keys_to_device(pos))
-    gh = f.evictionHistogram()
+    gh = f.evictionHistogram().astype(np.int64)
     of = O.Filter(oc, CAP)
-    occ_o, ok_o, chain = of.insert_many(pos, want_chain=True)
-    oh = np.zeros_like(gh)
-    np.add.at(oh, chain[ok_o & (chain > 0)].astype(np.int64), 1)
-    oh[-1] = int((~ok_o).sum())
+    if path == "binned":
+        occ_o, ok_o, chain = of.insert_many_phased(pos)
+    else:
+        occ_o, ok_o, chain = of.insert_many(pos, want_chain=True)
+    oh = _oracle_hist(ok_o, chain, gh.size)
     fail_g, fail_o = n - occ, n - occ_o
     assert int(gh[-1]) == fail_g and int(gh[1:].sum()) <= n
     cg, co = _coarse(gh), _coarse(oh)
@@ -150,12 +172,15 @@ def test_sweep_point_against_oracle(cf, oracle_mod, kw, eviction):
     mean_o = float((oh[1:-1] * lens[1:]).sum()) / max(ev_o, 1)
     fp_g = int(f.containsMany(cf.keys_to_device(neg)).sum().item())
     fp_o = int(of.contains_many(neg).sum())
-    print(f"{kw} {eviction}: failed {fail_g}/{fail_o}, evicting {ev_g}/{ev_o}, mean chain "
+    print(f"{kw} {eviction} {path}: failed {fail_g}/{fail_o}, evicting {ev_g}/{ev_o}, mean chain "
           f"{mean_g:.2f}/{mean_o:.2f}, bins {cg.tolist()}/{co.tolist()}, false positives {fp_g}/{fp_o}",
           flush=True)
-    assert abs(fail_g - fail_o) <= 4 * np.sqrt(fail_g + fail_o) + 0.25 * max(fail_g, fail_o) + 3
     pooled = (fp_g + fp_o) / (2 * neg.size)
-    assert abs(fp_g - fp_o) <= 4.5 * np.sqrt(2 * neg.size * pooled * (1 - pooled)) + 3
-    assert abs(ev_g - ev_o) <= 0.2 * ev_o + 5 * np.sqrt(ev_g + ev_o) + 3, (ev_g, ev_o)
-    assert abs(mean_g - mean_o) <= 0.25 * mean_o + 0.3, (mean_g, mean_o)
+    assert abs(fp_g - fp_o) <= 4.5 * np.sqrt(2 * neg.size * pooled * (1 - pooled)) + 3, (fp_g, fp_o)
+    assert abs(fail_g - fail_o) <= 4 * np.sqrt(fail_g + fail_o) + 0.25 * max(fail_g, fail_o) + 3, (fail_g, fail_o)
+    if path == "binned":
+        assert abs(ev_g - ev_o) <= 4.5 * np.sqrt(ev_g + ev_o) + 0.01 * ev_o + 3, (ev_g, ev_o)
+    else:
+        assert abs(ev_g - ev_o) <= 0.12 * ev_o + 3, (ev_g, ev_o)
+    assert abs(mean_g - mean_o) <= 0.35 * mean_o + 0.3, (mean_g, mean_o)
     f.close()

@@ -126,9 +126,14 @@ def test_presorted_flow_batch(cf, oracle_mod, kw):
     np.testing.assert_array_equal(_packed_host(cf, packed, c.bits_per_tag), np.sort(O.packed_tags(oc, nb, keys)))
     occ = f.insertPackedSorted(packed)
     assert occ >= n - 2, (occ, n)
-    assert_table_count(f, occ, n)
-    if occ == n:
-        assert f.containsMany(dk).all().item()
+    table = assert_table_count(f, occ, n)
+    missing = int((~f.containsMany(dk)).sum().item())
+    # Offset + BFS: after a lost swap the reference's clean-up removes the
+    # unflipped candidate tag (include/CuckooFilter.cuh:1011-1012), which can
+    # be another key's copy; each lost swap leaves a stray or such a removal
+    quirk = table - occ if kw.get("policy") == "offset" and kw.get("eviction") == "bfs" else 0
+    assert missing <= n - occ + quirk, (missing, n - occ, quirk)
+    if occ == n and missing == 0:
         out = torch.zeros(n // 2, dtype=torch.bool, device=dk.device)
         assert f.deleteMany(dk[: n // 2], out=out) == n - n // 2 and out.all().item()
     f.close()

@@ -113,7 +113,7 @@ class SlotModel:
                 return True
         return False
 
-    def bfs(self, fp, b):
+    def bfs_candidates(self, fp, b):
         for i in range(min(8, self.bs // 2)):
             s = (fp + i * 7) & (self.bs - 1)
             cand = self.t[b][s]
@@ -125,7 +125,10 @@ class SlotModel:
             if self.try_insert(ab, afp):
                 self.t[b][s] = fp
                 return True
-        return self.dfs(fp, b)
+        return False
+
+    def bfs(self, fp, b):
+        return self.bfs_candidates(fp, b) or self.dfs(fp, b)
 
     def insert(self, i1, fp):
         i2, fp2 = self.alt(i1, fp)
@@ -182,6 +185,42 @@ def test_sequential_insert_delete_matches_slot_model(oracle_mod, bits, bucket, w
     np.testing.assert_array_equal(dok, want_d)
     np.testing.assert_array_equal(O.table_tags(f.table_bytes(), cfg).astype(np.uint64), model.image_tags())
     assert occ2 == occ - int(want_d.sum()) == f.count_occupied_slots()
+
+
+@pytest.mark.parametrize("bits,bucket,wb,pol,ev,maxev,cap", TINY)
+def test_phased_insert_matches_slot_model(oracle_mod, bits, bucket, wb, pol, ev, maxev, cap):
+    """insert_many_phased: the same steps per key, scheduled by step over the
+    batch -- all primary attempts, then secondaries, then BFS candidate loops,
+    then DFS chains, each in key order."""
+    O = oracle_mod
+    cfg = O.make_config(key_bytes=8, bits_per_tag=bits, bucket_size=bucket, word_bytes=wb, policy=pol,
+                        eviction=ev, max_evictions=maxev)
+    f = O.Filter(cfg, cap)
+    nb = f.num_buckets
+    m = SlotModel(O, cfg, nb)
+    n = int(f.capacity() * 1.08)
+    keys = O.gen_keys(2000 + bits + bucket, n)
+    i1, _, fp = O.candidates(cfg, nb, keys)
+    occ, ok, _ = f.insert_many_phased(keys)
+    done = [m.try_insert(int(a), int(b)) for a, b in zip(i1, fp)]
+    start = []
+    for j in range(n):
+        b, t = int(i1[j]), int(fp[j])
+        b2, t2 = m.alt(b, t)
+        if not done[j]:
+            done[j] = m.try_insert(b2, t2)
+        start.append((t, b) if (t & 1) == 0 else (t2, b2))
+    if ev == O.EVICT_BFS:
+        for j in range(n):
+            if not done[j]:
+                done[j] = m.bfs_candidates(*start[j])
+    for j in range(n):
+        if not done[j]:
+            done[j] = m.dfs(*start[j])
+    np.testing.assert_array_equal(ok, np.array(done))
+    np.testing.assert_array_equal(O.table_tags(f.table_bytes(), cfg).astype(np.uint64), m.image_tags())
+    assert occ == int(ok.sum()) == f.count_occupied_slots()
+    assert (~ok).sum() > 0
 
 
 def test_sorted_insert_is_sequential_insert_in_packed_order(oracle_mod):
