@@ -86,7 +86,28 @@ SharedWorkspace& sharedWorkspace() {
 
 }  // namespace
 
+// ---- scratch of cf_insert_many_noalloc ------------------------------------
+// A file-scope device array: every device the library runs on holds its own
+// copy from the module's load until the process ends.  Host code never uses
+// the array's name as an address (on the host it only stands in for the
+// symbol); hipGetSymbolAddress gives each device's copy, looked up once per
+// device on that device's first no-allocation call, and only that address
+// reaches the kernels.
+static __device__ __align__(256) unsigned char scratch[CF_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+
+struct NoAllocScratch {
+    std::mutex mu;  // held by a call from launch to completion: calls on one process serialize
+    void* ptr[kMaxDevices] = {};
+};
+
+NoAllocScratch& noAllocScratch() {
+    static NoAllocScratch s;
+    return s;
+}
 
 int readOccupied(cf_filter* f, hipStream_t s, size_t* out) {
     if (out == nullptr) return CF_OK;
@@ -212,6 +233,41 @@ int cf_insert_many(cf_filter* f, const void* d_keys, size_t n, void* stream, siz
         if (e == hipSuccess) e = hipStreamSynchronize(s);
     }
     if (e != hipSuccess) return hipFail(e, "insertMany");
+    return readOccupied(f, s, occupied);
+}
+
+int cf_insert_many_noalloc(cf_filter* f, const void* d_keys, size_t n, void* stream, size_t* occupied) {
+    int r = checkFilter(f);
+    if (r == CF_OK) r = checkKeys(d_keys, n);
+    if (r != CF_OK) return r;
+    if (n > CF_NOALLOC_MAX_KEYS)
+        return fail(CF_ERR_INVALID, "cf_insert_many_noalloc takes at most %zu keys per call, got %zu",
+                    static_cast<size_t>(CF_NOALLOC_MAX_KEYS), n);
+    if (f->device < 0 || f->device >= kMaxDevices) return fail(CF_ERR_INVALID, "device %d out of range", f->device);
+    const size_t need = f->ops->insertWorkspaceBytes(n, f->numBuckets);
+    if (need > CF_NOALLOC_SCRATCH_BYTES)
+        return fail(CF_ERR_INVALID, "this insert needs %zu bytes of scratch; cf_insert_many_noalloc has %zu", need,
+                    static_cast<size_t>(CF_NOALLOC_SCRATCH_BYTES));
+    DeviceGuard g(f->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    NoAllocScratch& ns = noAllocScratch();
+    hipError_t e;
+    {
+        std::lock_guard<std::mutex> lock(ns.mu);
+        if (ns.ptr[f->device] == nullptr) {
+            void* p = nullptr;
+            e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess) return hipFail(e, "hipGetSymbolAddress(scratch)");
+            ns.ptr[f->device] = p;
+        }
+        cuckoo::Workspace ws;
+        ws.ptr = ns.ptr[f->device];
+        ws.bytes = CF_NOALLOC_SCRATCH_BYTES;
+        ws.borrowed = true;
+        e = f->ops->insert(f->view(), d_keys, n, f->cfg.block_size, s, ws);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    if (e != hipSuccess) return hipFail(e, "insertMany (no allocation)");
     return readOccupied(f, s, occupied);
 }
 

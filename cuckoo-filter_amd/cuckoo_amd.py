@@ -22,6 +22,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cuckoo_filter.h")
 
 CF_OK, CF_ERR_INVALID, CF_ERR_UNSUPPORTED, CF_ERR_HIP, CF_ERR_ALLOC = 0, -1, -2, -3, -4
 POLICIES = {"xor": 0, "addsub": 1, "offset": 2}
+CF_NOALLOC_MAX_KEYS = 1 << 21  # include/cuckoo_filter.h
 EVICTIONS = {"bfs": 0, "dfs": 1}
 
 
@@ -103,6 +104,7 @@ def _load():
         "cf_create": (i32, [cfgp, sz, C.POINTER(P)]),
         "cf_destroy": (i32, [P]),
         "cf_insert_many": (i32, [P, P, sz, P, C.POINTER(sz)]),
+        "cf_insert_many_noalloc": (i32, [P, P, sz, P, C.POINTER(sz)]),
         "cf_insert_many_sorted": (i32, [P, P, sz, P, C.POINTER(sz)]),
         "cf_contains_many": (i32, [P, P, sz, P, P]),
         "cf_delete_many": (i32, [P, P, sz, P, P, C.POINTER(sz)]),
@@ -269,6 +271,15 @@ class CuckooFilter:
         occ = C.c_size_t()
         _check(lib.cf_insert_many(self._h, C.c_void_p(k.data_ptr()), k.numel(), _stream_ptr(stream),
                                   C.byref(occ)))
+        return occ.value
+
+    def insertManyNoAlloc(self, keys, stream=None) -> int:
+        """insertMany without any device allocation (cf_insert_many_noalloc):
+        scratch comes from the library's fixed per-device array."""
+        k = self._keys(keys)
+        occ = C.c_size_t()
+        _check(lib.cf_insert_many_noalloc(self._h, C.c_void_p(k.data_ptr()), k.numel(), _stream_ptr(stream),
+                                          C.byref(occ)))
         return occ.value
 
     def insertManySorted(self, keys, stream=None) -> int:

@@ -86,6 +86,21 @@ int cf_destroy(cf_filter* f);
 /* size_t insertMany(const T* d_keys, size_t n, cudaStream_t)  (CuckooFilter.cuh:457-464) */
 int cf_insert_many(cf_filter* f, const void* d_keys, size_t n, void* stream, size_t* occupied);
 
+/* No counterpart in the reference.  cf_insert_many for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same insert and result, but every
+ * buffer beyond the filter, the keys and the counter read-back comes from a
+ * fixed device array of CF_NOALLOC_SCRATCH_BYTES that the library holds on
+ * each device it runs on until the process ends; the call allocates nothing.
+ * At most CF_NOALLOC_MAX_KEYS keys per call (CF_ERR_INVALID beyond that, or
+ * when a table is so large that the insert's slice index would not fit).
+ * Calls in one process serialize on a lock for the array; the array keeps the
+ * last call's contents, and calls that overlap on one GPU from different
+ * processes do not share it (each process has its own). */
+#define CF_NOALLOC_MAX_KEYS (1u << 21)
+#define CF_NOALLOC_SCRATCH_BYTES (64u << 20)
+int cf_insert_many_noalloc(cf_filter* f, const void* d_keys, size_t n, void* stream, size_t* occupied);
+
 /* size_t insertManySorted(const T* d_keys, size_t n, cudaStream_t)  (CuckooFilter.cuh:475-508) */
 int cf_insert_many_sorted(cf_filter* f, const void* d_keys, size_t n, void* stream, size_t* occupied);
 
