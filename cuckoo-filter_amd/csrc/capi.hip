@@ -306,8 +306,13 @@ int cf_delete_many(cf_filter* f, const void* d_keys, size_t n, bool* d_output, v
     if (r != CF_OK) return r;
     DeviceGuard g(f->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipError_t e = f->ops->remove(f->view(), d_keys, n, d_output, f->cfg.block_size, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipError_t e;
+    {
+        SharedWorkspace& sw = sharedWorkspace();
+        std::lock_guard<std::mutex> lock(sw.mu);
+        e = f->ops->remove(f->view(), d_keys, n, d_output, f->cfg.block_size, s, sw.perDevice[f->device]);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
     if (e != hipSuccess) return hipFail(e, "deleteMany");
     return readOccupied(f, s, occupied);
 }

@@ -77,8 +77,8 @@ struct Entry {
         return E::contains(view(a), static_cast<const Key*>(keys), n, out, block, s);
     }
     static hipError_t remove(const cfabi::AnyView& a, const void* keys, size_t n, bool* out, uint32_t block,
-                             hipStream_t s) {
-        return E::remove(view(a), static_cast<const Key*>(keys), n, out, block, s);
+                             hipStream_t s, cuckoo::Workspace& ws) {
+        return E::remove(view(a), static_cast<const Key*>(keys), n, out, block, s, ws);
     }
     static hipError_t candidates(const void* keys, size_t n, size_t nb, uint64_t* i1, uint64_t* i2, uint32_t* fp,
                                  uint32_t block, hipStream_t s) {

@@ -36,7 +36,8 @@ struct Ops {
     hipError_t (*insertSorted)(const AnyView&, const void* keys, size_t n, uint32_t block, hipStream_t,
                                cuckoo::Workspace&);
     hipError_t (*contains)(const AnyView&, const void* keys, size_t n, bool* out, uint32_t block, hipStream_t);
-    hipError_t (*remove)(const AnyView&, const void* keys, size_t n, bool* out, uint32_t block, hipStream_t);
+    hipError_t (*remove)(const AnyView&, const void* keys, size_t n, bool* out, uint32_t block, hipStream_t,
+                         cuckoo::Workspace&);
     hipError_t (*candidates)(const void* keys, size_t n, size_t nb, uint64_t* i1, uint64_t* i2, uint32_t* fp,
                              uint32_t block, hipStream_t);
     hipError_t (*packedTags)(const void* keys, size_t n, size_t nb, void* out, uint32_t block, hipStream_t);

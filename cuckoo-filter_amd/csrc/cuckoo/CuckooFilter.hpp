@@ -242,7 +242,7 @@ struct CuckooFilter {
 
     /// Removes one copy of each key's fingerprint; d_output (optional) gets per-key success.
     size_t deleteMany(const T* d_keys, const size_t n, bool* d_output = nullptr, hipStream_t stream = {}) {
-        HIP_CALL((cuckoo::Engine<Config>::remove(view(), d_keys, n, d_output, blockSize, stream)));
+        HIP_CALL((cuckoo::Engine<Config>::remove(view(), d_keys, n, d_output, blockSize, stream, workspace_)));
         HIP_CALL(hipStreamSynchronize(stream));
         return occupiedSlots();
     }

@@ -431,6 +431,255 @@ __global__ __launch_bounds__(kMaxBlock) void dfsStageKernel(const uint64_t* __re
     if (v.evictions != nullptr) detail::blockCountAdd(ev, v.evictions, false);
 }
 
+// ---------------------------------------------------------------------------
+// Bucket-binned delete (deleteMany, CuckooFilter.cuh:536-548, 1106-1116,
+// 1157-1183).  The same partition groups the keys' (primary bucket,
+// fingerprint) entries by table slice; one workgroup per slice removes every
+// entry's tag from its primary bucket inside LDS, the entries not found there
+// try their secondary bucket with device-scope CAS, and the entries not found
+// in either are the failed deletes.  Running all first removals, then all
+// second ones, is an interleaving the concurrent deleteKernel allows.
+//
+// Results: the entries carry no key index, so a delete's result cannot be
+// stored at its key's place by the slice that decides it (and random byte
+// stores would cost more than the delete itself).  Instead the failed
+// deletes' (primary bucket, fingerprint) pairs go into a hash table, and one
+// streaming pass over the keys writes true for every key except those that
+// claim one of the table's entries.  Keys with equal pairs are
+// interchangeable -- their deletes remove the same tag from the same two
+// buckets -- so which of them reports the failure is one of the orders the
+// concurrent reference allows.  With no failures the pass only writes true.
+// ---------------------------------------------------------------------------
+
+/// Step 2 of the delete: first-bucket removals of one slice's entries in LDS;
+/// entries not found go to `overflow` as (bucket << 32 | fingerprint).
+template <typename Config, typename E = uint64_t>
+__global__ __launch_bounds__(kBinBlock) void binDeleteKernel(const E* __restrict__ entries,
+                                                             const uint32_t* __restrict__ start, int log2Slice,
+                                                             DeviceView<Config> v, uint64_t* __restrict__ overflow,
+                                                             unsigned long long* overflowCount) {
+    using Ops = FilterOps<Config>;
+    using Word = typename Config::WordType;
+    using Tag = typename Config::TagType;
+    constexpr size_t WC = Layout<Config>::wordCount;
+    constexpr size_t kBucketBytes = Layout<Config>::bucketBytes;
+    constexpr int kTagBits = 8 * sizeof(Tag);
+    extern __shared__ __align__(16) unsigned char ldsRaw[];
+    __shared__ uint32_t ovCount;
+    __shared__ unsigned long long ovBase;
+
+    const uint32_t s = blockIdx.x;
+    const uint32_t lo = start[s];
+    const uint32_t hi = start[s + 1];
+    if (lo == hi) return;
+
+    const size_t sliceBuckets = static_cast<size_t>(1) << log2Slice;
+    Word* slice = reinterpret_cast<Word*>(ldsRaw);
+    uint64_t* ovBuf = reinterpret_cast<uint64_t*>(ldsRaw + binOverflowOffset(sliceBuckets * kBucketBytes));
+    const size_t b0 = static_cast<size_t>(s) << log2Slice;
+    const size_t nbs = v.numBuckets - b0 < sliceBuckets ? v.numBuckets - b0 : sliceBuckets;
+    const size_t words = nbs * WC;
+    Word* gw = v.words + b0 * WC;
+
+    if (threadIdx.x == 0) ovCount = 0;
+    const uint32_t last = hi - 1;
+    E next[kBinDepth];
+    auto loadRound = [&](uint32_t r0) {
+#pragma unroll
+        for (uint32_t k = 0; k < kBinDepth; ++k) {
+            const uint32_t e = r0 + k * kBinBlock + threadIdx.x;
+            next[k] = entries[e < last ? e : last];
+        }
+    };
+    loadRound(lo);
+    using V4 = unsigned int __attribute__((ext_vector_type(4)));
+    const size_t vecs = words * sizeof(Word) / 16;
+    for (size_t i = threadIdx.x; i < vecs; i += kBinBlock) {
+        reinterpret_cast<V4*>(slice)[i] = reinterpret_cast<const V4*>(gw)[i];
+    }
+    for (size_t i = vecs * 16 / sizeof(Word) + threadIdx.x; i < words; i += kBinBlock) slice[i] = gw[i];
+    __syncthreads();
+
+    uint32_t ok = 0;
+    const int lane = __lane_id();
+    const uint64_t ltMask = (1ull << lane) - 1;
+    for (uint32_t r0 = lo; r0 < hi; r0 += kBinRound) {
+        E ent[kBinDepth];
+#pragma unroll
+        for (uint32_t k = 0; k < kBinDepth; ++k) ent[k] = next[k];
+        loadRound(r0 + kBinRound < hi ? r0 + kBinRound : r0);
+#pragma unroll
+        for (uint32_t k = 0; k < kBinDepth; ++k) {
+            const bool valid = r0 + k * kBinBlock + threadIdx.x < hi;
+            const Tag fp = static_cast<Tag>(ent[k]);
+            uint32_t local;
+            if constexpr (sizeof(E) == 4) {
+                local = static_cast<uint32_t>(ent[k] >> kTagBits);
+            } else {
+                local = static_cast<uint32_t>(static_cast<size_t>(ent[k] >> 32) - b0);
+            }
+            bool removed = true;
+            if (valid) {
+                Word* bw = slice + static_cast<size_t>(local) * WC;
+                removed = Ops::tryRemoveSnap(bw, detail::loadBucket<Word, WC>(bw), fp);
+                ok += removed;
+            }
+            const uint64_t misses = __ballot(!removed);
+            if (misses != 0) {
+                const int leader = __ffsll(static_cast<long long>(misses)) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&ovCount, static_cast<uint32_t>(__popcll(misses)));
+                base = __shfl(base, leader, 64);
+                const uint32_t mine = base + static_cast<uint32_t>(__popcll(misses & ltMask));
+                if (!removed) {
+                    const uint64_t entry = ((b0 + local) << 32) | static_cast<uint64_t>(fp);
+                    if (mine < kOverflowLds) {
+                        ovBuf[mine] = entry;
+                    } else {
+                        overflow[atomicAdd(overflowCount, 1ull)] = entry;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t buffered = ovCount < kOverflowLds ? ovCount : kOverflowLds;
+    if (threadIdx.x == 0 && buffered != 0) ovBase = atomicAdd(overflowCount, static_cast<unsigned long long>(buffered));
+    for (size_t i = threadIdx.x; i < vecs; i += kBinBlock) {
+        reinterpret_cast<V4*>(gw)[i] = reinterpret_cast<const V4*>(slice)[i];
+    }
+    for (size_t i = vecs * 16 / sizeof(Word) + threadIdx.x; i < words; i += kBinBlock) gw[i] = slice[i];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < buffered; i += kBinBlock) overflow[ovBase + i] = ovBuf[i];
+    detail::blockCountAdd(ok, v.occupied, true);
+}
+
+/// Step 3 of the delete: the secondary-bucket removal of each overflow entry
+/// (i1 << 32 | fp); entries found in neither bucket go to `failed` unchanged.
+template <typename Config>
+__global__ __launch_bounds__(kStageBlock) void secondaryDeleteKernel(const uint64_t* __restrict__ in,
+                                                                     const unsigned long long* __restrict__ inCount,
+                                                                     uint64_t* __restrict__ failed,
+                                                                     unsigned long long* failedCount,
+                                                                     DeviceView<Config> v) {
+    using Ops = FilterOps<Config>;
+    using Tag = typename Config::TagType;
+    __shared__ uint64_t buf[kStageChunk];
+    __shared__ uint32_t bufCount;
+    __shared__ unsigned long long base;
+    const size_t n = static_cast<size_t>(*inCount);
+    uint32_t ok = 0;
+    for (size_t c0 = static_cast<size_t>(blockIdx.x) * kStageChunk; c0 < n;
+         c0 += static_cast<size_t>(gridDim.x) * kStageChunk) {
+        if (threadIdx.x == 0) bufCount = 0;
+        __syncthreads();
+        const size_t c1 = n - c0 < kStageChunk ? n : c0 + kStageChunk;
+        for (size_t i = c0 + threadIdx.x; i < c1; i += kStageBlock) {
+            const uint64_t entry = in[i];
+            const size_t i1 = static_cast<size_t>(entry >> 32);
+            const Tag fp = static_cast<Tag>(entry);
+            const auto second = Ops::altWithNewFp(i1, fp, v.numBuckets);
+            const bool removed = Ops::tryRemoveAtBucket(v, second.bucket, second.fp);
+            ok += removed;
+            waveAppend(!removed, entry, buf, &bufCount);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && bufCount != 0) base = atomicAdd(failedCount, static_cast<unsigned long long>(bufCount));
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < bufCount; j += kStageBlock) failed[base + j] = buf[j];
+        __syncthreads();
+    }
+    detail::blockCountAdd(ok, v.occupied, true);
+}
+
+/// Hash table of failed deletes: open addressing over 64-bit (i1 << 32 | fp)
+/// entries, one slot per failure (equal pairs take separate slots).  A
+/// fingerprint is never 0, so 0 marks an empty slot and a value with a zero
+/// fingerprint field marks a slot a key has claimed.
+inline constexpr uint64_t kFailEmpty = 0;
+inline constexpr uint64_t kFailClaimed = 0xFFFFFFFF00000000ull;
+
+/// Table slots for f failures: the power of two at or above 2f (at least 64).
+__device__ __forceinline__ uint64_t failTableSlots(unsigned long long f) {
+    uint64_t c = 64;
+    while (c < 2 * static_cast<uint64_t>(f)) c <<= 1;
+    return c;
+}
+
+__device__ __forceinline__ uint64_t failHash(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    return x;
+}
+
+/// (Templates only so that every translation unit may instantiate them.)
+template <typename C = unsigned long long>
+__global__ __launch_bounds__(256) void failTableClearKernel(uint64_t* __restrict__ table,
+                                                            const C* __restrict__ failedCount) {
+    const unsigned long long f = *failedCount;
+    if (f == 0) return;
+    const uint64_t slots = failTableSlots(f);
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < slots;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        table[i] = kFailEmpty;
+    }
+}
+
+template <typename C = unsigned long long>
+__global__ __launch_bounds__(256) void failTableBuildKernel(uint64_t* __restrict__ table,
+                                                            const uint64_t* __restrict__ failed,
+                                                            const C* __restrict__ failedCount) {
+    const unsigned long long f = *failedCount;
+    if (f == 0) return;
+    const uint64_t mask = failTableSlots(f) - 1;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < f;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t key = failed[i];
+        for (uint64_t h = failHash(key) & mask;; h = (h + 1) & mask) {
+            unsigned long long expected = kFailEmpty;
+            if (__hip_atomic_compare_exchange_strong(reinterpret_cast<unsigned long long*>(table + h), &expected,
+                                                     static_cast<unsigned long long>(key), __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                break;
+            }
+        }
+    }
+}
+
+/// deleteMany's per-key results: true, except for keys that claim a failed
+/// entry with their (i1, fp) -- as many keys per pair as deletes of that
+/// pair failed.
+template <typename Config>
+__global__ __launch_bounds__(256) void deleteResultKernel(const typename Config::KeyType* __restrict__ keys,
+                                                          size_t n, size_t numBuckets, bool* __restrict__ out,
+                                                          uint64_t* __restrict__ table,
+                                                          const unsigned long long* __restrict__ failedCount) {
+    const unsigned long long f = *failedCount;
+    const uint64_t mask = f != 0 ? failTableSlots(f) - 1 : 0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        bool ok = true;
+        if (f != 0) {
+            const auto c = Config::AltBucketPolicy::getCandidateBuckets(keys[i], numBuckets);
+            const uint64_t key = (static_cast<uint64_t>(c.i1) << 32) | static_cast<uint64_t>(c.fp);
+            for (uint64_t h = failHash(key) & mask;; h = (h + 1) & mask) {
+                unsigned long long cur = __hip_atomic_load(reinterpret_cast<unsigned long long*>(table + h),
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (cur == kFailEmpty) break;
+                if (cur == key &&
+                    __hip_atomic_compare_exchange_strong(reinterpret_cast<unsigned long long*>(table + h), &cur,
+                                                         kFailClaimed, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)) {
+                    ok = false;
+                    break;
+                }
+            }
+        }
+        out[i] = ok;
+    }
+}
+
 namespace detail {
 
 inline int log2Floor(size_t v) {
@@ -452,6 +701,16 @@ inline InsertPath insertPathFromEnv() {
     if (std::strcmp(e, "direct") == 0) return InsertPath::Direct;
     if (std::strcmp(e, "binned") == 0) return InsertPath::Binned;
     if (std::strcmp(e, "presorted") == 0) return InsertPath::Presorted;
+    return InsertPath::Auto;
+}
+
+/// Delete path selection: CUCKOO_DELETE_PATH = "direct" | "binned" | "auto"
+/// (default auto: binned from one key per kBinnedMinBucketsPerKey buckets).
+inline InsertPath deletePathFromEnv() {
+    const char* e = std::getenv("CUCKOO_DELETE_PATH");
+    if (e == nullptr) return InsertPath::Auto;
+    if (std::strcmp(e, "direct") == 0) return InsertPath::Direct;
+    if (std::strcmp(e, "binned") == 0) return InsertPath::Binned;
     return InsertPath::Auto;
 }
 
@@ -500,6 +759,13 @@ struct BinnedInsert {
                align256((numSlices(numBuckets) + 1) * sizeof(uint32_t)) + align256(3 * sizeof(unsigned long long));
     }
 
+    /// The delete's workspace with per-key results: the insert's, plus the
+    /// failed-delete table (a power of two of slots at or above 2x the
+    /// failures, at most 4n slots of 8 bytes).
+    static size_t deleteWorkspaceBytes(size_t n, size_t numBuckets, bool results) {
+        return workspaceBytes(n, numBuckets) + (results ? align256((4 * n + 64) * sizeof(uint64_t)) : 0);
+    }
+
     /// Whether the slice index splits into two digits: then the partition is
     /// two-level (MSD).  Where they fit in 32 bits its values are (bucket bits
     /// below the first digit, fingerprint) after the first pass and (bucket
@@ -515,15 +781,36 @@ struct BinnedInsert {
         return l2s + (sliceBits - radix::kDigitBits) + kTagBits <= 32;
     }
 
-    template <bool SortedSemantics>
-    static hipError_t run(const View& v, const Key* keys, size_t n, uint32_t block, hipStream_t s, Workspace& ws) {
+    /// The slice kernel of step 2: primary-bucket insert attempts, or
+    /// first-bucket removals for the delete.
+    template <bool Delete, typename E>
+    static void launchSlices(const E* entries, const uint32_t* start, int l2s, const View& v, uint64_t* overflow,
+                             unsigned long long* counters, size_t slices, hipStream_t s) {
+        const size_t sliceBytes = (size_t(1) << l2s) * kBucketBytes;
+        if constexpr (Delete) {
+            binDeleteKernel<Config, E><<<static_cast<uint32_t>(slices), kBinBlock, binRetryOffset(sliceBytes), s>>>(
+                entries, start, l2s, v, overflow, counters);
+        } else {
+            const size_t lds = binLdsBytes(sliceBytes, sizeof(BinRetryOf<Config>));
+            binInsertKernel<Config, E><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(entries, start, l2s, v,
+                                                                                           overflow, counters);
+        }
+    }
+
+    /// Bucket-binned insertMany / insertManySorted (Delete = false), or
+    /// deleteMany (Delete = true; `out` may be nullptr).
+    template <bool SortedSemantics, bool Delete = false>
+    static hipError_t run(const View& v, const Key* keys, size_t n, uint32_t block, hipStream_t s, Workspace& ws,
+                          bool* out = nullptr) {
         constexpr size_t kChunk = size_t(1) << 31;
         const int l2s = log2Slice(v.numBuckets);
         const size_t slices = numSlices(v.numBuckets);
         const int sliceBits = detail::bitsFor(slices);
         for (size_t off = 0; off < n; off += kChunk) {
             const size_t m = n - off < kChunk ? n - off : kChunk;
-            hipError_t e = ws.reserve(workspaceBytes(m, v.numBuckets));
+            const size_t need = Delete ? deleteWorkspaceBytes(m, v.numBuckets, out != nullptr)
+                                       : workspaceBytes(m, v.numBuckets);
+            hipError_t e = ws.reserve(need);
             if (e != hipSuccess) return e;
             char* p = static_cast<char*>(ws.ptr);
             uint64_t* a = reinterpret_cast<uint64_t*>(p);
@@ -534,15 +821,18 @@ struct BinnedInsert {
             p += align256(radix::twoLevelScratchBytes(m));
             uint32_t* start = reinterpret_cast<uint32_t*>(p);
             p += align256((slices + 1) * sizeof(uint32_t));
-            unsigned long long* counters = reinterpret_cast<unsigned long long*>(p);  // overflow, evict, dfs
+            unsigned long long* counters = reinterpret_cast<unsigned long long*>(p);  // overflow, evict/fail, dfs
+            p += align256(3 * sizeof(unsigned long long));
+            uint64_t* failTable = reinterpret_cast<uint64_t*>(p);  // delete with results only
 
             EntrySource<Config, SortedSemantics> src{keys + off, v.numBuckets};
-            const size_t lds = binLdsBytes((size_t(1) << l2s) * kBucketBytes, sizeof(BinRetryOf<Config>));
+            // after step 2 the overflow list is in `ov`; `other` is free
+            uint64_t* ov = nullptr;
+            uint64_t* other = nullptr;
             if (twoLevel(sliceBits) && !twoLevelPacked(l2s, sliceBits)) {
                 // the first pass keeps whole 64-bit entries in a; the second
                 // writes b, in 32 bits (the bucket's place in its slice beside
-                // the tag) where they fit, else whole entries; then the lists
-                // as below
+                // the tag) where they fit, else whole entries
                 const int d2 = sliceBits - radix::kDigitBits;
                 zeroCountersKernel<unsigned long long><<<1, 64, 0, s>>>(counters);
                 if (l2s + kTagBits <= 32) {
@@ -552,24 +842,19 @@ struct BinnedInsert {
                                                            32 + l2s, uint32_t(1) << d2, pack2, a, b32, scratch,
                                                            start, static_cast<uint32_t>(slices), s);
                     if (e != hipSuccess) return e;
-                    binInsertKernel<Config, uint32_t><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(
-                        b32, start, l2s, v, a, counters);
+                    launchSlices<Delete>(b32, start, l2s, v, a, counters, slices, s);
                 } else {
                     e = radix::partitionTwoLevel<uint64_t>(src, m, 32 + l2s + d2, radix::Identity<uint64_t>{},
                                                            32 + l2s, uint32_t(1) << d2, radix::Identity<uint64_t>{},
                                                            a, b, scratch, start, static_cast<uint32_t>(slices), s);
                     if (e != hipSuccess) return e;
-                    binInsertKernel<Config, uint64_t><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(
-                        b, start, l2s, v, a, counters);
+                    launchSlices<Delete>(b, start, l2s, v, a, counters, slices, s);
                 }
-                e = runStages(v, m, block, s, counters, a, b);
-                if (e != hipSuccess) return e;
-                continue;
-            }
-            if (twoLevel(sliceBits)) {
+                ov = a;
+                other = b;
+            } else if (twoLevel(sliceBits)) {
                 // a holds the first pass's 32-bit values, b the second's; after
-                // the primary attempts the overflow list goes to a, eviction
-                // starts to b, the DFS list to a
+                // the slices the overflow list goes to a
                 const int d2 = sliceBits - radix::kDigitBits;
                 const PackBelowDigit pack1{(uint64_t(1) << (l2s + d2)) - 1, kTagBits};
                 const KeepLowBits pack2{(uint32_t(1) << (l2s + kTagBits)) - 1};
@@ -580,30 +865,52 @@ struct BinnedInsert {
                                                        uint32_t(1) << d2, pack2, a32, b32, scratch, start,
                                                        static_cast<uint32_t>(slices), s);
                 if (e != hipSuccess) return e;
-                binInsertKernel<Config, uint32_t><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(
-                    b32, start, l2s, v, a, counters);
-                e = runStages(v, m, block, s, counters, a, b);
-                if (e != hipSuccess) return e;
-                continue;
-            }
-            uint64_t* sorted = a;
-            if (sliceBits > 0) {
-                e = radix::sortBits<uint64_t>(src, m, 32 + l2s, 32 + l2s + sliceBits, a, b, scratch, s, &sorted);
-                if (e != hipSuccess) return e;
+                launchSlices<Delete>(b32, start, l2s, v, a, counters, slices, s);
+                ov = a;
+                other = b;
             } else {
-                auto k = packEntriesKernel<decltype(src)>;
-                k<<<detail::gridFor(k, m, 256), 256, 0, s>>>(src, m, a);
+                uint64_t* sorted = a;
+                if (sliceBits > 0) {
+                    e = radix::sortBits<uint64_t>(src, m, 32 + l2s, 32 + l2s + sliceBits, a, b, scratch, s, &sorted);
+                    if (e != hipSuccess) return e;
+                } else {
+                    auto k = packEntriesKernel<decltype(src)>;
+                    k<<<detail::gridFor(k, m, 256), 256, 0, s>>>(src, m, a);
+                }
+                uint64_t* spare = sorted == a ? b : a;
+                sliceStartKernel<uint64_t><<<static_cast<uint32_t>((slices + 1 + 255) / 256), 256, 0, s>>>(
+                    sorted, static_cast<uint32_t>(m), 32 + l2s, static_cast<uint32_t>(slices), start, counters);
+                launchSlices<Delete>(sorted, start, l2s, v, spare, counters, slices, s);
+                ov = spare;
+                other = sorted;  // free once the slices have read it
             }
-            uint64_t* spare = sorted == a ? b : a;
-            sliceStartKernel<uint64_t><<<static_cast<uint32_t>((slices + 1 + 255) / 256), 256, 0, s>>>(
-                sorted, static_cast<uint32_t>(m), 32 + l2s, static_cast<uint32_t>(slices), start, counters);
-            // overflow list -> spare; eviction starts -> sorted (free by then); DFS list -> spare
-            binInsertKernel<Config><<<static_cast<uint32_t>(slices), kBinBlock, lds, s>>>(sorted, start, l2s, v,
-                                                                                       spare, counters);
-            e = runStages(v, m, block, s, counters, spare, sorted);
+            if constexpr (Delete) {
+                e = runDeleteStages(v, keys + off, m, s, counters, ov, other, failTable,
+                                    out != nullptr ? out + off : nullptr);
+            } else {
+                e = runStages(v, m, block, s, counters, ov, other);
+            }
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
+    }
+
+    /// The delete after its slices: secondary removals of the overflow list,
+    /// then (with `out`) the per-key results from the failed entries.
+    static hipError_t runDeleteStages(const View& v, const Key* keys, size_t m, hipStream_t s,
+                                      unsigned long long* counters, uint64_t* overflow, uint64_t* failed,
+                                      uint64_t* failTable, bool* out) {
+        const uint32_t stageGrid = detail::gridFor(secondaryDeleteKernel<Config>,
+                                                   (m + kStageChunk - 1) / kStageChunk * kStageBlock, kStageBlock);
+        secondaryDeleteKernel<Config><<<stageGrid, kStageBlock, 0, s>>>(overflow, counters, failed, counters + 1, v);
+        if (out == nullptr) return hipGetLastError();
+        auto clear = failTableClearKernel<unsigned long long>;
+        auto build = failTableBuildKernel<unsigned long long>;
+        clear<<<detail::gridFor(clear, 4 * m + 64, 256), 256, 0, s>>>(failTable, counters + 1);
+        build<<<detail::gridFor(build, m, 256), 256, 0, s>>>(failTable, failed, counters + 1);
+        auto k = deleteResultKernel<Config>;
+        k<<<detail::gridFor(k, m, 256), 256, 0, s>>>(keys, m, v.numBuckets, out, failTable, counters + 1);
+        return hipGetLastError();
     }
 
     /// Step 3 over the overflow list in `overflow`; `other` receives the

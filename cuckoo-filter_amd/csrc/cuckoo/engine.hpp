@@ -76,9 +76,25 @@ struct Engine {
         return hipGetLastError();
     }
 
-    static hipError_t remove(const View& v, const Key* keys, size_t n, bool* out, uint32_t block,
-                             hipStream_t s) {
+    static bool useBinnedDelete(size_t n, size_t numBuckets) {
+        if (!Binned::applicable(numBuckets) || n == 0) return false;
+        switch (detail::deletePathFromEnv()) {
+            case detail::InsertPath::Direct:
+                return false;
+            case detail::InsertPath::Binned:
+                return true;
+            default:
+                return n * detail::kBinnedMinBucketsPerKey >= numBuckets;
+        }
+    }
+
+    /// deleteMany (CuckooFilter.cuh:536-548): the bucket-binned path for
+    /// batches dense enough to pay for staging table slices, one thread per
+    /// key with device-scope CAS otherwise.
+    static hipError_t remove(const View& v, const Key* keys, size_t n, bool* out, uint32_t block, hipStream_t s,
+                             Workspace& ws) {
         if (n == 0) return hipSuccess;
+        if (useBinnedDelete(n, v.numBuckets)) return Binned::template run<false, true>(v, keys, n, block, s, ws, out);
         auto k = deleteKernel<Config>;
         k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, out, n, v);
         return hipGetLastError();
