@@ -235,7 +235,8 @@ def batched_section(cf, torch, workloads, dev, n_filters=256, n_batches=8192, se
     return out
 
 
-def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(0.5, 0.7, 1.0)):
+def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(0.5, 0.7, 1.0),
+                    batched_mkeys_s=None, capped_levels=(), host_burst_run=True):
     """Serving workload: canonical 31-mers of a 2^28-base ACGT sequence (10%
     of it in duplicated regions), 2-bit packed on the device, in one 2^30-slot
     filter, plus 64 tenant filters of 2^20..2^24 slots (log-uniform, filled to
@@ -398,19 +399,37 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
         "insert_keys_clipped": clipped,
         "kmers": n_kmers,
     }
-    with cf.Server() as srv:
+    mean_keys = total_keys / n_requests
+
+    def poisson(mkeys_s):
+        rate = mkeys_s * 1e6 / mean_keys  # requests per second
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+        return arr - arr[0]
+
+    def sweep(srv, into):
         run(srv, np.zeros(n_requests))  # warm-up: plans, graphs, arenas
         sat = run(srv, np.zeros(n_requests))
-        out["saturated"] = sat
-        peak = sat["mkeys_s"] * 1e6
-        mean_keys = total_keys / n_requests
+        into["saturated"] = sat
         for frac in loads:
-            rate = frac * peak / mean_keys  # requests per second
-            arr = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
-            arr -= arr[0]
-            out[f"load_{int(frac * 100)}pct"] = dict(run(srv, arr), offered_mkeys_s=round(frac * peak / 1e6, 1))
-    with cf.Server(window_slots=3) as srv3:  # a third slot overlaps host copies with uploads
-        out["host_memory_burst"] = dict(host_burst(srv3), window_slots=3)
+            offered = frac * sat["mkeys_s"]
+            into[f"load_{int(frac * 100)}pct"] = dict(run(srv, poisson(offered)), offered_mkeys_s=round(offered, 1))
+        if batched_mkeys_s:
+            # the load the serving target names: 70% of the batched window's throughput
+            offered = 0.7 * batched_mkeys_s
+            into["load_70pct_of_batched"] = dict(run(srv, poisson(offered)), offered_mkeys_s=round(offered, 1))
+
+    with cf.Server() as srv:
+        sweep(srv, out)
+    for cap in capped_levels:
+        # windows capped at `cap` levels: shorter windows, so a request waits
+        # for less work ahead of it
+        part = {"max_window_levels": cap}
+        with cf.Server(max_window_levels=cap) as srv:
+            sweep(srv, part)
+        out[f"max_window_levels_{cap}"] = part
+    if host_burst_run:
+        with cf.Server(window_slots=3) as srv3:  # a third slot overlaps host copies with uploads
+            out["host_memory_burst"] = dict(host_burst(srv3), window_slots=3)
     for x in filters + snapshots:
         x.close()
     del kmers, pools, lpools, ring
@@ -601,7 +620,8 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_serving:
         log("serving loop: saturated and open-loop latency")
-        result["serving"] = serving_section(cf, torch, workloads, dev)
+        batched_rate = result.get("batched", {}).get("batched_mkeys_s")
+        result["serving"] = serving_section(cf, torch, workloads, dev, batched_mkeys_s=batched_rate)
         torch.cuda.empty_cache()
     if not args.no_roofline:
         import roofline as rl

@@ -210,6 +210,9 @@ struct cf_server {
     std::thread launcher;
     std::thread completer;
 
+    uint32_t maxLevels = 0;  // levels per window (0 in the options: unlimited)
+    std::unordered_map<const cf_filter*, std::pair<int32_t, int32_t>> depth;  // launcher's planning scratch
+
     void launcherLoop();
     void completerLoop();
     int launchWindow(Slot& sl);
@@ -302,11 +305,26 @@ void cf_server::launcherLoop() {
         sl.ops = queue.front().req.filter->ops;
         size_t keys = 0;
         size_t stage = 0;
+        // the level each request would take in the window (the batch planner's
+        // rule: a lookup one level after its filter's last insert / delete, an
+        // insert / delete one level after its filter's last request)
+        depth.clear();
         while (!queue.empty() && sl.reqs.size() < opt.max_window_requests) {
             const Pending& p = queue.front();
             if (p.req.filter->ops != sl.ops) break;  // one configuration per window
             if (!sl.reqs.empty() && keys + p.req.n > opt.max_window_keys) break;
             if (stage + p.stageKeys + p.stageOut > arenaBytes) break;  // waits for the arena
+            auto [it, fresh] = depth.try_emplace(p.req.filter, std::pair<int32_t, int32_t>{-1, -1});
+            (void)fresh;
+            auto& [lastMutation, maxLevel] = it->second;
+            const bool lookup = p.req.op == CF_OP_LOOKUP;
+            const int32_t level = lookup ? lastMutation + 1 : maxLevel + 1;
+            if (!sl.reqs.empty() && static_cast<uint32_t>(level) >= maxLevels) break;
+            if (lookup) {
+                maxLevel = std::max(maxLevel, level);
+            } else {
+                lastMutation = maxLevel = level;
+            }
             keys += p.req.n;
             stage += p.stageKeys + p.stageOut;
             sl.reqs.push_back(p);
@@ -473,7 +491,7 @@ int cf_server_options_init(cf_server_options* o) {
     o->max_window_keys = 1ull << 22;
     o->staging_bytes = 0;
     o->window_slots = 2;
-    o->reserved = 0;
+    o->max_window_levels = 0;
     return CF_OK;
 }
 
@@ -490,10 +508,10 @@ int cf_server_create(const cf_server_options* o, cf_server** out) {
         return fail(CF_ERR_INVALID, "window limits must be positive");
     if (opt.window_slots < 2 || opt.window_slots > kMaxSlots)
         return fail(CF_ERR_INVALID, "window_slots must be 2..%d (got %u)", kMaxSlots, opt.window_slots);
-    if (opt.reserved != 0) return fail(CF_ERR_INVALID, "reserved options field must be 0");
     auto* s = new (std::nothrow) cf_server();
     if (s == nullptr) return fail(CF_ERR_ALLOC, "host allocation failed");
     s->opt = opt;
+    s->maxLevels = opt.max_window_levels != 0 ? opt.max_window_levels : UINT32_MAX;
     DeviceGuard g(opt.device);
     hipError_t e = hipSuccess;
     size_t staging = opt.staging_bytes;

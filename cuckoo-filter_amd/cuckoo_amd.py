@@ -56,7 +56,7 @@ REQUEST_DTYPE = np.dtype([("filter", "u8"), ("op", "u4"), ("flags", "u4"), ("key
 RESULT_DTYPE = np.dtype([("status", "i4"), ("window_levels", "u4"), ("window", "u8"), ("occupied", "u8"),
                          ("arrival_ns", "u8"), ("start_ns", "u8"), ("done_ns", "u8")])
 SERVER_OPTIONS_DTYPE = np.dtype([("device", "i4"), ("max_window_requests", "u4"), ("max_window_keys", "u8"),
-                                 ("staging_bytes", "u8"), ("window_slots", "u4"), ("reserved", "u4")])
+                                 ("staging_bytes", "u8"), ("window_slots", "u4"), ("max_window_levels", "u4")])
 SERVER_STATS_DTYPE = np.dtype([(k, "u8") for k in ("windows", "requests", "keys", "levels", "staging_bytes",
                                                     "max_queue")])
 assert REQUEST_DTYPE.itemsize == 40 and RESULT_DTYPE.itemsize == 48
@@ -497,7 +497,7 @@ class Server:
     running one; 3 lets host-memory staging overlap the uploads."""
 
     def __init__(self, device=None, max_window_requests=None, max_window_keys=None, staging_bytes=None,
-                 window_slots=None):
+                 window_slots=None, max_window_levels=None):
         o = np.zeros(1, SERVER_OPTIONS_DTYPE)
         _check(lib.cf_server_options_init(o.ctypes.data))
         if device is not None:
@@ -510,6 +510,8 @@ class Server:
             o["staging_bytes"] = staging_bytes
         if window_slots is not None:
             o["window_slots"] = window_slots
+        if max_window_levels is not None:
+            o["max_window_levels"] = max_window_levels
         h = C.c_void_p()
         _check(lib.cf_server_create(o.ctypes.data, C.byref(h)))
         self._h = h

@@ -237,7 +237,10 @@ typedef struct cf_server_options {
                                      0 = min(512 MiB, 1/16 of free HBM at creation) */
     uint32_t window_slots;        /* windows formed ahead: 2..8, default 2; 3 overlaps the
                                      staging of host-memory windows with their uploads */
-    uint32_t reserved;            /* must be 0 */
+    uint32_t max_window_levels;   /* 0 = unlimited (default); else a window stops taking requests
+                                     before one that would need more levels than this (a level
+                                     per insert / delete on one filter, and one per run of
+                                     lookups between them), which bounds a window's duration */
 } cf_server_options;
 
 typedef struct cf_request {

@@ -135,12 +135,12 @@ def test_struct_layouts_match_header(cfm, tmp_path):
             assert got[(name, fld)] == getattr(st, fld).offset, (name, fld)
 
 
-@pytest.mark.parametrize("field,value", [("window_slots", 1), ("window_slots", 9), ("reserved", 1),
+@pytest.mark.parametrize("field,value", [("window_slots", 1), ("window_slots", 9), ("max_window_requests", 0),
                                          ("max_window_keys", 0)])
 def test_server_options_rejected_before_any_device_call(cfm, field, value):
     o = np.zeros(1, cfm.SERVER_OPTIONS_DTYPE)
     cfm.lib.cf_server_options_init(o.ctypes.data)
-    assert int(o["window_slots"][0]) == 2 and int(o["reserved"][0]) == 0
+    assert int(o["window_slots"][0]) == 2 and int(o["max_window_levels"][0]) == 0
     o[field] = value
     h = ctypes.c_void_p()
     rc = cfm.lib.cf_server_create(o.ctypes.data, ctypes.byref(h))

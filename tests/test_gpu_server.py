@@ -93,7 +93,10 @@ def test_server_lookups_alone_and_in_mix_bit_exact(cf, oracle_mod):
         assert (res["done_ns"] >= res["start_ns"]).all() and (res["start_ns"] >= res["arrival_ns"]).all()
 
 
-def test_server_keeps_arrival_order_per_filter(cf, oracle_mod):
+@pytest.mark.parametrize("max_levels", [None, 1, 2])
+def test_server_keeps_arrival_order_per_filter(cf, oracle_mod, max_levels):
+    """Also with windows capped at one and two levels: the cap splits the
+    stream into more windows, never reorders it."""
     import torch
 
     f = cf.CuckooFilter(1 << 16, key_bytes=8, bits_per_tag=16, eviction="bfs")
@@ -104,9 +107,13 @@ def test_server_keeps_arrival_order_per_filter(cf, oracle_mod):
             (g, cf.OP_INSERT, k, None, False), (f, cf.OP_LOOKUP, k, outs[1], False),
             (f, cf.OP_LOOKUP, k, outs[2], False), (f, cf.OP_DELETE, k, None, False),
             (f, cf.OP_LOOKUP, k, outs[3], False)]
-    with cf.Server() as srv:
+    with cf.Server(max_window_levels=max_levels) as srv:
         res = srv.replay(srv.make_requests(reqs), np.zeros(len(reqs)))
+        windows = srv.stats()["windows"]
     assert (res["status"] == 0).all()
+    if max_levels is not None:
+        assert int(res["window_levels"].max()) <= max_levels
+        assert windows >= {1: 5, 2: 3}[max_levels]
     assert not outs[0].any().item()
     assert outs[1].all().item() and outs[2].all().item()
     assert not outs[3].any().item()
@@ -114,8 +121,8 @@ def test_server_keeps_arrival_order_per_filter(cf, oracle_mod):
     assert f.occupiedSlots() == 0 and g.occupiedSlots() == 30000
 
 
-@pytest.mark.parametrize("staging", [None, 1 << 20])
-def test_server_mixed_stream_matches_plain(cf, oracle_mod, staging):
+@pytest.mark.parametrize("staging,max_levels", [(None, None), (1 << 20, None), (None, 3)])
+def test_server_mixed_stream_matches_plain(cf, oracle_mod, staging, max_levels):
     import torch
 
     O = oracle_mod
@@ -179,10 +186,12 @@ def test_server_mixed_stream_matches_plain(cf, oracle_mod, staging):
             out = torch.zeros(max(keys.size, 1), dtype=torch.bool, device="cuda") if kind else None
         reqs.append((served[i], kind, k, out, host))
     arrivals = np.cumsum(rng.exponential(1e-4, size=len(reqs)))
-    with cf.Server(staging_bytes=staging) as srv:
+    with cf.Server(staging_bytes=staging, max_window_levels=max_levels) as srv:
         res = srv.replay(srv.make_requests(reqs), arrivals)
         st = srv.stats()
     assert (res["status"] == 0).all()
+    if max_levels is not None:
+        assert int(res["window_levels"].max()) <= max_levels
     np.testing.assert_array_equal(res["occupied"], np.array(occ_plain, dtype=np.uint64))
     for (i, kind, keys, want, _), (_, _, _, out, _), op in zip(plan, reqs, out_plain):
         if not keys.size or kind == 0:
