@@ -54,16 +54,47 @@ BYTES_PER_KEY = {
 }
 
 
+def sources_digest():
+    """SHA-256 (first 16 hex digits) of the HIP sources the library is built
+    from, in path order: what a profile's counters were measured on."""
+    import hashlib
+
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "cuckoo-filter_amd", "csrc")
+    files = []
+    for d, _, names in os.walk(csrc):
+        files += [os.path.join(d, x) for x in names if x.endswith((".hip", ".hpp", ".h"))]
+    for path in sorted(files, key=lambda p: os.path.relpath(p, csrc)):
+        h.update(os.path.relpath(path, csrc).encode())
+        with open(path, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def _traffic_profile():
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            return fh.read()
+    except OSError:
+        return ""
+
+
 def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE + WRITE_SIZE
     summary (counters scaled by a plain copy of known size), or None."""
-    try:
-        with open(TRAFFIC_PROFILE) as fh:
-            text = fh.read()
-    except OSError:
-        return None
-    m = re.search(rf"^traffic {re.escape(kernel)} (\d+)\s*$", text, re.M)
+    m = re.search(rf"^traffic {re.escape(kernel)} (\d+)\s*$", _traffic_profile(), re.M)
     return int(m.group(1)) if m else None
+
+
+def traffic_stamp():
+    """The git revision and source digest the traffic summary was measured at,
+    and whether the sources are still those."""
+    text = _traffic_profile()
+    rev = re.search(r"^revision (\S+)\s*$", text, re.M)
+    dig = re.search(r"^sources (\S+)\s*$", text, re.M)
+    return {"traffic_revision": rev.group(1) if rev else None,
+            "traffic_sources": dig.group(1) if dig else None,
+            "traffic_current": bool(dig) and dig.group(1) == sources_digest()}
 
 
 def _roofline(kernel, bytes_per_key, n, ms):
@@ -83,6 +114,7 @@ def _roofline(kernel, bytes_per_key, n, ms):
         "algorithmic_bytes": int(bytes_per_key * n),
         "bytes_per_key": round(bytes_per_key, 2),
         "ms_per_launch": round(ms, 3),
+        **traffic_stamp(),
     }
 
 
@@ -436,6 +468,58 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
     return out
 
 
+def big_table_section(cf, torch, workloads, dev, log2_slots=30, reps=3):
+    """The plain step on a table eight times the memory-side cache: 2^30 slots
+    (2^26 buckets x 32 B = 2 GiB), 16-bit tags, bucket 16, u64 words, BFS.
+    insertMany to 95% load (1,020,054,732 keys), containsMany of 2^27 keys
+    (50/50) and deleteMany of a quarter of the inserted keys, each the mean
+    of `reps` timed runs after one untimed, with HIP events on the launch
+    stream.  Algorithmic bytes per key as for the 2^28-slot step."""
+    cap = 1 << log2_slots
+    f = cf.CuckooFilter(cap, key_bytes=8, bits_per_tag=16, bucket_size=16, word_bytes=8, policy="xor",
+                        eviction="bfs", max_evictions=500, block_size=256)
+    n = int(cap * LOAD)
+    nd = n // 4
+    log(f"2^{log2_slots}-slot table: {n} inserts, {N_LOOKUP} lookups, {nd} deletes")
+    keys = workloads.gen_keys_torch(SEED + 1, n + N_LOOKUP // 2, device=dev)
+    query = torch.cat([keys[: N_LOOKUP // 2], keys[n:]])
+    out = torch.empty(N_LOOKUP, dtype=torch.bool, device=dev)
+    dout = torch.empty(nd, dtype=torch.bool, device=dev)
+    t = {"insert": [], "lookup": [], "delete": []}
+    occ = rem = 0
+    for i in range(reps + 1):
+        f.clear()
+        ev = {k: _events(torch) for k in t}
+        ev["insert"][0].record()
+        occ = f.insertMany(keys[:n])
+        ev["insert"][1].record()
+        ev["lookup"][0].record()
+        f.containsMany(query, out=out)
+        ev["lookup"][1].record()
+        ev["delete"][0].record()
+        rem = f.deleteMany(keys[:nd], out=dout)
+        ev["delete"][1].record()
+        torch.cuda.synchronize()
+        if i:
+            for k in t:
+                t[k].append(ev[k][0].elapsed_time(ev[k][1]))
+    half = N_LOOKUP // 2
+    res = {"slots": cap, "table_bytes": f.sizeInBytes(), "keys_inserted": n, "lookups": N_LOOKUP,
+           "deletes": nd, "placed": occ, "after_delete": rem,
+           "positives_found": int(out[:half].sum().item()), "false_positives": int(out[half:].sum().item()),
+           "deletes_ok": bool(dout.all().item())}
+    units = {"insert": n, "lookup": N_LOOKUP, "delete": nd}
+    for k, v in t.items():
+        ms = sum(v) / len(v)
+        res[f"{k}_ms"] = round(ms, 3)
+        res[f"{k}_mkeys_s"] = round(units[k] / ms / 1e3, 1)
+        res[f"{k}_algorithmic_gbs"] = round(BYTES_PER_KEY[k] * units[k] / (ms / 1e3) / 1e9, 1)
+    del keys, query, out, dout
+    f.close()
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -449,6 +533,9 @@ def main():
     ap.add_argument("--no-roofline", action="store_true", help="skip the measured gather/CAS roofline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-filter comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency runs")
+    ap.add_argument("--no-big-table", action="store_true", help="skip the 2^30-slot (2 GiB) table")
+    ap.add_argument("--big-table-only", action="store_true",
+                    help="run only the 2^30-slot table's insert / lookup / delete (for profiling it alone)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the cuckoo-filter hot path has no multi-GPU leg in this build "
@@ -462,6 +549,9 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.big_table_only:
+        print(json.dumps({"table_2gib": big_table_section(cf, torch, workloads, dev)}), flush=True)
+        return 0
     n_ins = int(CAPACITY * LOAD)
     n_del = n_ins // 4
     f = cf.CuckooFilter(CAPACITY, key_bytes=8, bits_per_tag=16, bucket_size=16, word_bytes=8,
@@ -578,6 +668,10 @@ def main():
             load_sweep[str(alpha)] = row
         per_op["insert_sorted_mkeys_s"] = load_sweep["0.95"]["insert_sorted_mkeys_s"]
 
+    big_table = None
+    if not args.no_big_table:
+        big_table = big_table_section(cf, torch, workloads, dev)
+
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -611,6 +705,8 @@ def main():
     }
     if load_sweep is not None:
         result["load_sweep"] = load_sweep
+    if big_table is not None:
+        result["table_2gib"] = big_table
     del keys, pos, query, dels, out, dout
     f.close()
     torch.cuda.empty_cache()
@@ -635,6 +731,10 @@ def main():
                 result["breakdown"][f"{op}_frac_of_gather32"] = round(per_op[f"{op}_algorithmic_gbs"] / g, 3)
             for r in (roofline, roofline_insert):
                 r["frac_of_measured_gather32"] = round(r["achieved"] / g, 3)
+        g2 = m.get("gather32_2048MiB_gbs")
+        if g2 and big_table is not None:
+            for op in ("insert", "lookup", "delete"):
+                big_table[f"{op}_frac_of_gather32_2048MiB"] = round(big_table[f"{op}_algorithmic_gbs"] / g2, 3)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline")
         result["cpu_baseline"] = cpu_baseline(n_ins, N_LOOKUP, n_del,

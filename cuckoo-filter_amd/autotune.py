@@ -255,6 +255,14 @@ def batched_trial(kw, n_filters=256, n_batches=8192, seed=7, device="cuda", mix=
         occ_p = run_plain()
         torch.cuda.synchronize()
         t_plain = time.perf_counter() - t0
+        # the same plain calls from the library's C loop (no Python per call)
+        parr = b.make_ops([(plain[i], kind, keys, out) for i, kind, keys, out in ops_spec])
+        restore(plain)
+        t0 = time.perf_counter()
+        occ_c = cf.run_plain_calls(parr)
+        torch.cuda.synchronize()
+        t_plain_c = time.perf_counter() - t0
+        same_c = all(int(occ_c[k]) == occ_p[k] for k in range(len(ops_spec)) if ops_spec[k][1] != 1)
     t_batched = float("inf")
     for _ in range(max(1, repeats)):
         restore(batched)
@@ -274,11 +282,14 @@ def batched_trial(kw, n_filters=256, n_batches=8192, seed=7, device="cuda", mix=
                      f"{n_batches} batches log-uniform 32..2^20 keys, insert/lookup/delete mix {mix}"),
         "keys": total_keys,
         "plain_mkeys_s": round(total_keys / t_plain / 1e6, 1),
+        "plain_c_loop_mkeys_s": round(total_keys / t_plain_c / 1e6, 1),
         "batched_mkeys_s": round(total_keys / t_batched / 1e6, 1),
         "speedup": round(t_plain / t_batched, 2),
+        "speedup_vs_c_loop": round(t_plain_c / t_batched, 2),
         "plain_ms": round(t_plain * 1e3, 2),
+        "plain_c_loop_ms": round(t_plain_c * 1e3, 2),
         "batched_ms": round(t_batched * 1e3, 2),
-        "occupancies_match_plain": bool(same),
+        "occupancies_match_plain": bool(same and same_c),
     }
 
 

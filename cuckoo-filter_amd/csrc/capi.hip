@@ -584,6 +584,32 @@ int cf_batch_execute(cf_batch* b, const cf_batch_op* ops, size_t count, void* st
     return e == hipSuccess ? CF_OK : hipFail(e, "batch execute");
 }
 
+int cf_run_plain_calls(const cf_batch_op* ops, size_t count, void* stream, uint64_t* h_occupied) {
+    if (count != 0 && ops == nullptr) return fail(CF_ERR_INVALID, "ops is NULL");
+    for (size_t i = 0; i < count; ++i) {
+        const cf_batch_op& o = ops[i];
+        size_t occ = 0;
+        int r;
+        switch (o.op) {
+            case CF_OP_INSERT:
+                r = cf_insert_many(o.filter, o.d_keys, o.n, stream, &occ);
+                break;
+            case CF_OP_LOOKUP:
+                r = cf_contains_many(o.filter, o.d_keys, o.n, o.d_output, stream);
+                if (r == CF_OK) r = cf_occupied_slots(o.filter, &occ);
+                break;
+            case CF_OP_DELETE:
+                r = cf_delete_many(o.filter, o.d_keys, o.n, o.d_output, stream, &occ);
+                break;
+            default:
+                return fail(CF_ERR_INVALID, "op %zu: unknown operation %u", i, o.op);
+        }
+        if (r != CF_OK) return r;
+        if (h_occupied != nullptr) h_occupied[i] = occ;
+    }
+    return CF_OK;
+}
+
 static_assert(sizeof(cf_table_header) == 80, "table file header layout");
 
 int cf_save_table(cf_filter* f, const char* path) {

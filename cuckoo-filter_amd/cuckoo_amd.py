@@ -129,6 +129,7 @@ def _load():
         "cf_batch_create": (i32, [C.POINTER(P)]),
         "cf_batch_destroy": (i32, [P]),
         "cf_batch_execute": (i32, [P, P, sz, P, P]),
+        "cf_run_plain_calls": (i32, [P, sz, P, P]),
         "cf_save_table": (i32, [P, C.c_char_p]),
         "cf_load_table": (i32, [C.c_char_p, C.POINTER(P)]),
         "cf_kmer_file_write": (i32, [C.c_char_p, P, sz]),
@@ -475,6 +476,15 @@ class Batch:
         _check(lib.cf_batch_execute(self._h, arr, len(arr), _stream_ptr(stream),
                                     occ.ctypes.data if occ is not None else None))
         return occ
+
+
+def run_plain_calls(ops, stream=None, want_occupied=True):
+    """ops (a Batch.make_ops array, or (filter, op, keys, out) tuples) one at a
+    time through the plain calls, from the library's C loop."""
+    arr = ops if isinstance(ops, C.Array) else Batch.make_ops(ops)
+    occ = np.zeros(len(arr), np.uint64) if want_occupied else None
+    _check(lib.cf_run_plain_calls(arr, len(arr), _stream_ptr(stream), occ.ctypes.data if occ is not None else None))
+    return occ
 
 
 def _ptr(buf) -> int:

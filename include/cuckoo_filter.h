@@ -208,6 +208,14 @@ int cf_batch_destroy(cf_batch* b);
  * receives each op's post-op occupancy -- what the plain call returns. */
 int cf_batch_execute(cf_batch* b, const cf_batch_op* ops, size_t count, void* stream, uint64_t* h_occupied);
 
+/* The same ops run one at a time through cf_insert_many / cf_contains_many /
+ * cf_delete_many, in array order, from a C loop: the per-call pattern of the
+ * reference's API (launch, synchronise, read the counter back;
+ * CuckooFilter.cuh:457-548) without a scripting layer's overhead -- the
+ * baseline cf_batch_execute replaces.  h_occupied (optional) receives each
+ * op's post-op occupancy (for lookups, the occupancy at that point). */
+int cf_run_plain_calls(const cf_batch_op* ops, size_t count, void* stream, uint64_t* h_occupied);
+
 /* ------------------------------------------------------------------------ */
 /* Serving loop (no counterpart in the reference; its nearest analogue is    */
 /* the synchronous one-worker IPC server, include/CuckooFilterIPC.cuh:200-307) */

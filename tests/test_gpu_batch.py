@@ -191,9 +191,23 @@ def test_batch_mixed_window_matches_plain(cf, oracle_mod):
         ops_b.append((batched[i], int(kind), dk, out_b))
         want = np.array([int(x) in present[i] for x in keys], dtype=bool) if kind == 1 else None
         checks.append((int(kind), keys, want))
+    # a third twin of every filter runs the window through cf_run_plain_calls (the C loop)
+    looped = []
+    for b in batched:
+        t = cf.CuckooFilter(b.capacity(), config=b.config)
+        t.copyFrom(b)
+        looped.append(t)
+    by_filter = {id(b): t for b, t in zip(batched, looped)}
+    ops_c = [(by_filter[id(f)], kind, dk, torch.zeros_like(ob) if ob is not None else None)
+             for f, kind, dk, ob in ops_b]
     occ_p = _run_plain(ops_p)
     occ_b = cf.Batch().execute(ops_b)
+    occ_c = cf.run_plain_calls(ops_c)
     np.testing.assert_array_equal(occ_b, occ_p)
+    np.testing.assert_array_equal(occ_c, occ_p)
+    for (_, kind, dk, oc_), (_, _, _, op) in zip(ops_c, ops_p):
+        if kind == 2 and dk.numel():
+            assert oc_[: dk.numel()].all().item()
     for (f, kind, dk, op), (_, _, _, ob), (_, keys, want_pos) in zip(ops_p, ops_b, checks):
         if kind == 2 and keys.size:
             assert ob[: keys.size].all().item() and op[: keys.size].all().item()
