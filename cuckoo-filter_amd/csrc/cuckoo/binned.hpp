@@ -121,23 +121,84 @@ __global__ __launch_bounds__(256) void sliceStartKernel(const T* __restrict__ en
     start[s] = lo;
 }
 
-/// Overflow entries a workgroup buffers in LDS before reserving their global
-/// space with a single atomic (more than this spill with one atomic per wave).
-inline constexpr uint32_t kOverflowLds = 2048;
-
 /// Entries each thread of binInsertKernel loads at once, and the entries a
 /// workgroup takes per round.
 inline constexpr uint32_t kBinDepth = 4;
 inline constexpr uint32_t kBinRound = kBinBlock * kBinDepth;
 
+/// Dynamic LDS one slice workgroup may take so that two stay resident on a
+/// CU (160 KiB): half of it, less room for the kernels' static counters.
+inline constexpr size_t kBinLdsBudget = 80 * 1024 - 256;
+
+/// Overflow entries a workgroup buffers in LDS before reserving their global
+/// space with a single atomic: what the budget leaves beside the slice (and,
+/// in the insert, beside a round's retry list of `retryBytes` entries).
+inline constexpr uint32_t binOverflowCapacity(size_t retryBytes) {
+    return static_cast<uint32_t>((kBinLdsBudget - kSliceBytes - kBinRound * retryBytes) / sizeof(uint64_t));
+}
+
 /// Dynamic LDS of binInsertKernel: the table slice, the overflow buffer, then
 /// the round's retry list (kBinRound entries of retryBytes each).
 inline constexpr size_t binOverflowOffset(size_t sliceBytes) { return (sliceBytes + 15) / 16 * 16; }
-inline constexpr size_t binRetryOffset(size_t sliceBytes) {
-    return binOverflowOffset(sliceBytes) + kOverflowLds * sizeof(uint64_t);
+inline constexpr size_t binRetryOffset(size_t sliceBytes, size_t retryBytes) {
+    return binOverflowOffset(sliceBytes) + binOverflowCapacity(retryBytes) * sizeof(uint64_t);
 }
-inline size_t binLdsBytes(size_t sliceBytes, size_t retryBytes) {
-    return binRetryOffset(sliceBytes) + kBinRound * retryBytes;
+inline constexpr size_t binLdsBytes(size_t sliceBytes, size_t retryBytes) {
+    return binRetryOffset(sliceBytes, retryBytes) + kBinRound * retryBytes;
+}
+/// Dynamic LDS of binDeleteKernel: the table slice, then its overflow buffer.
+inline constexpr size_t binDeleteLdsBytes(size_t sliceBytes) {
+    return binOverflowOffset(sliceBytes) + binOverflowCapacity(0) * sizeof(uint64_t);
+}
+static_assert(binLdsBytes(kSliceBytes, 8) <= kBinLdsBudget && binDeleteLdsBytes(kSliceBytes) <= kBinLdsBudget);
+static_assert(binOverflowCapacity(8) >= 2000);
+
+/// Appends the lanes' entries with `emit` to the workgroup's LDS overflow
+/// buffer of `capacity` entries (one LDS atomic per wave).  Entries past it
+/// go straight to the global list, with one global atomic per wave for the
+/// wave's whole run of them.
+__device__ __forceinline__ void appendOverflow(bool emit, uint64_t entry, uint64_t* ovBuf, uint32_t* ovCount,
+                                               uint32_t capacity, uint64_t* overflow,
+                                               unsigned long long* overflowCount) {
+    const uint64_t mask = __ballot(emit);
+    if (mask == 0) return;
+    const int lane = __lane_id();
+    const uint64_t ltMask = (1ull << lane) - 1;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(ovCount, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader, 64);
+    const uint32_t mine = base + static_cast<uint32_t>(__popcll(mask & ltMask));
+    const bool spill = emit && mine >= capacity;
+    if (emit && !spill) ovBuf[mine] = entry;
+    const uint64_t spills = __ballot(spill);
+    if (spills == 0) return;
+    const int sl = __ffsll(static_cast<long long>(spills)) - 1;
+    unsigned long long gbase = 0;
+    if (lane == sl) gbase = atomicAdd(overflowCount, static_cast<unsigned long long>(__popcll(spills)));
+    gbase = __shfl(gbase, sl, 64);
+    if (spill) overflow[gbase + __popcll(spills & ltMask)] = entry;
+}
+
+/// Between two rounds of a slice kernel (every thread, after a barrier):
+/// once the LDS overflow buffer is half full, moves its entries to the global
+/// list with one global atomic.  Slices of small buckets near full load
+/// overflow by thousands of entries; reserving their space lane by lane
+/// serialised the whole grid on the list counter's address, and flushing
+/// after every round costs three barriers a round where few entries fail.
+__device__ __forceinline__ void flushOverflowBetweenRounds(uint64_t* ovBuf, uint32_t* ovCount, uint32_t capacity,
+                                                           uint64_t* overflow, unsigned long long* overflowCount,
+                                                           unsigned long long* ovBase) {
+    const uint32_t total = *ovCount;  // buffered entries plus those appendOverflow spilled past the buffer
+    if (total < capacity / 2) return;  // uniform: every thread reads the same count
+    const uint32_t count = total < capacity ? total : capacity;  // the spilled ones are on the list already
+    if (threadIdx.x == 0) *ovBase = atomicAdd(overflowCount, static_cast<unsigned long long>(count));
+    __syncthreads();
+    const unsigned long long base = *ovBase;
+    for (uint32_t i = threadIdx.x; i < count; i += kBinBlock) overflow[base + i] = ovBuf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *ovCount = 0;
+    __syncthreads();
 }
 
 /// Retry-list entry of binInsertKernel: (bucket inside the slice << tag bits
@@ -187,7 +248,8 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict
     const size_t sliceBuckets = static_cast<size_t>(1) << log2Slice;
     Word* slice = reinterpret_cast<Word*>(ldsRaw);
     uint64_t* ovBuf = reinterpret_cast<uint64_t*>(ldsRaw + binOverflowOffset(sliceBuckets * kBucketBytes));
-    R* retry = reinterpret_cast<R*>(ldsRaw + binRetryOffset(sliceBuckets * kBucketBytes));
+    R* retry = reinterpret_cast<R*>(ldsRaw + binRetryOffset(sliceBuckets * kBucketBytes, sizeof(R)));
+    constexpr uint32_t kOverflowCap = binOverflowCapacity(sizeof(R));
     const size_t b0 = static_cast<size_t>(s) << log2Slice;
     const size_t nbs = v.numBuckets - b0 < sliceBuckets ? v.numBuckets - b0 : sliceBuckets;
     const size_t words = nbs * WC;
@@ -273,27 +335,15 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict
                 placed = Ops::tryInsertSnap(bw, detail::loadBucket<Word, WC>(bw), fp);
                 ok += placed;
             }
-            const uint64_t fails = __ballot(!placed);
-            if (fails != 0) {
-                // one LDS atomic per wave reserves this wave's run in the overflow buffer
-                const int leader = __ffsll(static_cast<long long>(fails)) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(&ovCount, static_cast<uint32_t>(__popcll(fails)));
-                base = __shfl(base, leader, 64);
-                const uint32_t mine = base + static_cast<uint32_t>(__popcll(fails & ltMask));
-                if (!placed) {
-                    const uint64_t entry = ((b0 + local) << 32) | static_cast<uint64_t>(fp);
-                    if (mine < kOverflowLds) {
-                        ovBuf[mine] = entry;
-                    } else {
-                        overflow[atomicAdd(overflowCount, 1ull)] = entry;  // buffer full (rare)
-                    }
-                }
-            }
+            appendOverflow(!placed, ((b0 + local) << 32) | static_cast<uint64_t>(fp), ovBuf, &ovCount,
+                           kOverflowCap, overflow, overflowCount);
         }
         __syncthreads();  // the retry list is refilled by the next round
+        if (r0 + kBinRound < hi) {
+            flushOverflowBetweenRounds(ovBuf, &ovCount, kOverflowCap, overflow, overflowCount, &ovBase);
+        }
     }
-    const uint32_t buffered = ovCount < kOverflowLds ? ovCount : kOverflowLds;
+    const uint32_t buffered = ovCount < kOverflowCap ? ovCount : kOverflowCap;
     if (threadIdx.x == 0 && buffered != 0) ovBase = atomicAdd(overflowCount, static_cast<unsigned long long>(buffered));
     for (size_t i = threadIdx.x; i < vecs; i += kBinBlock) {
         reinterpret_cast<V4*>(gw)[i] = reinterpret_cast<const V4*>(slice)[i];
@@ -476,6 +526,7 @@ __global__ __launch_bounds__(kBinBlock) void binDeleteKernel(const E* __restrict
     const size_t sliceBuckets = static_cast<size_t>(1) << log2Slice;
     Word* slice = reinterpret_cast<Word*>(ldsRaw);
     uint64_t* ovBuf = reinterpret_cast<uint64_t*>(ldsRaw + binOverflowOffset(sliceBuckets * kBucketBytes));
+    constexpr uint32_t kOverflowCap = binOverflowCapacity(0);
     const size_t b0 = static_cast<size_t>(s) << log2Slice;
     const size_t nbs = v.numBuckets - b0 < sliceBuckets ? v.numBuckets - b0 : sliceBuckets;
     const size_t words = nbs * WC;
@@ -501,8 +552,6 @@ __global__ __launch_bounds__(kBinBlock) void binDeleteKernel(const E* __restrict
     __syncthreads();
 
     uint32_t ok = 0;
-    const int lane = __lane_id();
-    const uint64_t ltMask = (1ull << lane) - 1;
     for (uint32_t r0 = lo; r0 < hi; r0 += kBinRound) {
         E ent[kBinDepth];
 #pragma unroll
@@ -524,26 +573,16 @@ __global__ __launch_bounds__(kBinBlock) void binDeleteKernel(const E* __restrict
                 removed = Ops::tryRemoveSnap(bw, detail::loadBucket<Word, WC>(bw), fp);
                 ok += removed;
             }
-            const uint64_t misses = __ballot(!removed);
-            if (misses != 0) {
-                const int leader = __ffsll(static_cast<long long>(misses)) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(&ovCount, static_cast<uint32_t>(__popcll(misses)));
-                base = __shfl(base, leader, 64);
-                const uint32_t mine = base + static_cast<uint32_t>(__popcll(misses & ltMask));
-                if (!removed) {
-                    const uint64_t entry = ((b0 + local) << 32) | static_cast<uint64_t>(fp);
-                    if (mine < kOverflowLds) {
-                        ovBuf[mine] = entry;
-                    } else {
-                        overflow[atomicAdd(overflowCount, 1ull)] = entry;
-                    }
-                }
-            }
+            appendOverflow(!removed, ((b0 + local) << 32) | static_cast<uint64_t>(fp), ovBuf, &ovCount,
+                           kOverflowCap, overflow, overflowCount);
+        }
+        if (r0 + kBinRound < hi) {
+            __syncthreads();
+            flushOverflowBetweenRounds(ovBuf, &ovCount, kOverflowCap, overflow, overflowCount, &ovBase);
         }
     }
     __syncthreads();
-    const uint32_t buffered = ovCount < kOverflowLds ? ovCount : kOverflowLds;
+    const uint32_t buffered = ovCount < kOverflowCap ? ovCount : kOverflowCap;
     if (threadIdx.x == 0 && buffered != 0) ovBase = atomicAdd(overflowCount, static_cast<unsigned long long>(buffered));
     for (size_t i = threadIdx.x; i < vecs; i += kBinBlock) {
         reinterpret_cast<V4*>(gw)[i] = reinterpret_cast<const V4*>(slice)[i];
@@ -788,7 +827,7 @@ struct BinnedInsert {
                              unsigned long long* counters, size_t slices, hipStream_t s) {
         const size_t sliceBytes = (size_t(1) << l2s) * kBucketBytes;
         if constexpr (Delete) {
-            binDeleteKernel<Config, E><<<static_cast<uint32_t>(slices), kBinBlock, binRetryOffset(sliceBytes), s>>>(
+            binDeleteKernel<Config, E><<<static_cast<uint32_t>(slices), kBinBlock, binDeleteLdsBytes(sliceBytes), s>>>(
                 entries, start, l2s, v, overflow, counters);
         } else {
             const size_t lds = binLdsBytes(sliceBytes, sizeof(BinRetryOf<Config>));
@@ -835,15 +874,20 @@ struct BinnedInsert {
                 // the tag) where they fit, else whole entries
                 const int d2 = sliceBits - radix::kDigitBits;
                 zeroCountersKernel<unsigned long long><<<1, 64, 0, s>>>(counters);
-                if (l2s + kTagBits <= 32) {
-                    const PackBelowDigit pack2{(uint64_t(1) << l2s) - 1, kTagBits};
-                    uint32_t* b32 = reinterpret_cast<uint32_t*>(b);
-                    e = radix::partitionTwoLevel<uint64_t>(src, m, 32 + l2s + d2, radix::Identity<uint64_t>{},
-                                                           32 + l2s, uint32_t(1) << d2, pack2, a, b32, scratch,
-                                                           start, static_cast<uint32_t>(slices), s);
-                    if (e != hipSuccess) return e;
-                    launchSlices<Delete>(b32, start, l2s, v, a, counters, slices, s);
-                } else {
+                bool packedSecond = false;
+                if constexpr (kTagBits < 32) {  // 32-bit tags never fit beside a bucket index in 32 bits
+                    if (l2s + kTagBits <= 32) {
+                        const PackBelowDigit pack2{(uint64_t(1) << l2s) - 1, kTagBits};
+                        uint32_t* b32 = reinterpret_cast<uint32_t*>(b);
+                        e = radix::partitionTwoLevel<uint64_t>(src, m, 32 + l2s + d2, radix::Identity<uint64_t>{},
+                                                               32 + l2s, uint32_t(1) << d2, pack2, a, b32, scratch,
+                                                               start, static_cast<uint32_t>(slices), s);
+                        if (e != hipSuccess) return e;
+                        launchSlices<Delete>(b32, start, l2s, v, a, counters, slices, s);
+                        packedSecond = true;
+                    }
+                }
+                if (!packedSecond) {
                     e = radix::partitionTwoLevel<uint64_t>(src, m, 32 + l2s + d2, radix::Identity<uint64_t>{},
                                                            32 + l2s, uint32_t(1) << d2, radix::Identity<uint64_t>{},
                                                            a, b, scratch, start, static_cast<uint32_t>(slices), s);
@@ -852,22 +896,24 @@ struct BinnedInsert {
                 }
                 ov = a;
                 other = b;
-            } else if (twoLevel(sliceBits)) {
+            } else if (kTagBits < 32 && twoLevel(sliceBits)) {
                 // a holds the first pass's 32-bit values, b the second's; after
                 // the slices the overflow list goes to a
-                const int d2 = sliceBits - radix::kDigitBits;
-                const PackBelowDigit pack1{(uint64_t(1) << (l2s + d2)) - 1, kTagBits};
-                const KeepLowBits pack2{(uint32_t(1) << (l2s + kTagBits)) - 1};
-                uint32_t* a32 = reinterpret_cast<uint32_t*>(a);
-                uint32_t* b32 = reinterpret_cast<uint32_t*>(b);
-                zeroCountersKernel<unsigned long long><<<1, 64, 0, s>>>(counters);
-                e = radix::partitionTwoLevel<uint64_t>(src, m, 32 + l2s + d2, pack1, l2s + kTagBits,
-                                                       uint32_t(1) << d2, pack2, a32, b32, scratch, start,
-                                                       static_cast<uint32_t>(slices), s);
-                if (e != hipSuccess) return e;
-                launchSlices<Delete>(b32, start, l2s, v, a, counters, slices, s);
-                ov = a;
-                other = b;
+                if constexpr (kTagBits < 32) {
+                    const int d2 = sliceBits - radix::kDigitBits;
+                    const PackBelowDigit pack1{(uint64_t(1) << (l2s + d2)) - 1, kTagBits};
+                    const KeepLowBits pack2{(uint32_t(1) << (l2s + kTagBits)) - 1};
+                    uint32_t* a32 = reinterpret_cast<uint32_t*>(a);
+                    uint32_t* b32 = reinterpret_cast<uint32_t*>(b);
+                    zeroCountersKernel<unsigned long long><<<1, 64, 0, s>>>(counters);
+                    e = radix::partitionTwoLevel<uint64_t>(src, m, 32 + l2s + d2, pack1, l2s + kTagBits,
+                                                           uint32_t(1) << d2, pack2, a32, b32, scratch, start,
+                                                           static_cast<uint32_t>(slices), s);
+                    if (e != hipSuccess) return e;
+                    launchSlices<Delete>(b32, start, l2s, v, a, counters, slices, s);
+                    ov = a;
+                    other = b;
+                }
             } else {
                 uint64_t* sorted = a;
                 if (sliceBits > 0) {

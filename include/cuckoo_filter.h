@@ -341,7 +341,10 @@ int cf_kmer_file_read(const char* path, uint64_t* h_keys, size_t n);
  * device: d_out[i] = key of seq[i .. i+k-1], len - k + 1 keys.  A=0 C=1 G=2
  * T=3, first base in the highest bits (src/kmc_to_binary.cpp:17-46); with
  * `canonical` the smaller of the k-mer and its reverse complement; a k-mer
- * holding another character gets the key UINT64_MAX. */
+ * holding another character gets the key UINT64_MAX.  That marker is also
+ * the key of the valid k-mer of 32 T's when k = 32 and `canonical` is 0 (no
+ * other k and no canonical key reaches it): callers that must tell the two
+ * apart check the sequence for that run of T's. */
 int cf_encode_kmers(const void* d_seq, size_t len, uint32_t k, int canonical, uint64_t* d_out, void* stream);
 
 #ifdef __cplusplus
