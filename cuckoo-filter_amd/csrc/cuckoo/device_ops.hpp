@@ -176,6 +176,25 @@ struct FilterOps {
     /// Buckets up to 128 B are handled as register snapshots; larger ones word by word.
     static constexpr bool kSnapshot = L::bucketBytes <= 128;
     using SnapT = detail::Snap<Word, WC>;
+    /// Lanes that read one bucket together in the lookup kernel: one per 32
+    /// bytes of 64- and 128-byte buckets.  Over tables of 4 GiB and more the
+    /// random reads are bound by address translation, once per load
+    /// instruction and lane, so a bucket read by one lane as four or eight
+    /// 16-byte loads costs two to three times what it costs spread over lanes.
+    static constexpr uint32_t kLookupLanes =
+        (kSnapshot && L::bucketBytes >= 64) ? static_cast<uint32_t>(L::bucketBytes / 32) : 1u;
+    /// Whether a lookup kernel of `block` threads reads buckets with
+    /// kLookupLanes lanes per key: always for 128-byte buckets (equal or
+    /// faster at every table size measured), for 64-byte buckets only past
+    /// 2 GiB of table (below that one lane per key is 1-6% faster).
+    static __host__ __device__ __forceinline__ bool groupedLookup(size_t numBuckets, uint32_t block) {
+        if constexpr (kLookupLanes == 1) {
+            return false;
+        } else {
+            return block % kLookupLanes == 0 &&
+                   (L::bucketBytes >= 128 || numBuckets * L::bucketBytes > (static_cast<size_t>(2) << 30));
+        }
+    }
 
     static __host__ __device__ __forceinline__ Tag extractTag(Word packed, size_t lane) {
         return static_cast<Tag>((packed >> (lane * BITS)) & L::fpMask);
@@ -342,6 +361,13 @@ struct FilterOps {
             }
             return false;
         }
+    }
+
+    /// Whether the 32-byte part `part` of bucket b holds the tag (one lane's
+    /// share of a bucket read by kLookupLanes lanes).
+    static __device__ __forceinline__ bool bucketPartHas(const View& v, size_t b, uint32_t part, Tag tag) {
+        constexpr size_t PW = 32 / sizeof(Word);
+        return chunkHas<PW>(detail::loadBucket<Word, PW>(bucketPtr(v, b) + part * PW), tag);
     }
 
     template <size_t N>

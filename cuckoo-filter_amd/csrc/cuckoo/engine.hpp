@@ -72,7 +72,9 @@ struct Engine {
                                hipStream_t s) {
         if (n == 0) return hipSuccess;
         auto k = containsKernel<Config>;
-        k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, out, n, v);
+        using Ops = FilterOps<Config>;
+        const size_t threads = Ops::groupedLookup(v.numBuckets, block) ? n * Ops::kLookupLanes : n;
+        k<<<detail::gridFor(k, threads, block), block, 0, s>>>(keys, out, n, v);
         return hipGetLastError();
     }
 

@@ -73,12 +73,55 @@ __global__ __launch_bounds__(kMaxBlock) void insertKernel(const typename Config:
     if (v.evictions != nullptr) detail::blockCountAdd(ev, v.evictions, false);
 }
 
-/// containsKernel (CuckooFilter.cuh:1143-1155).
+namespace detail {
+
+/// The lookup with kLookupLanes (L) consecutive lanes per key, each reading
+/// 32 bytes of the key's buckets; a ballot over the group gives the result,
+/// which the group's first lane stores.  Virtual thread t handles part t % L
+/// of key t / L; with blockDim a multiple of L a group never straddles a wave
+/// and every lane of a group takes the same branches.
+template <typename Config>
+__device__ __forceinline__ void containsGrouped(const typename Config::KeyType* __restrict__ keys,
+                                                bool* __restrict__ output, size_t n, const DeviceView<Config>& v) {
+    using Ops = FilterOps<Config>;
+    using Policy = typename Config::AltBucketPolicy;
+    using Tag = typename Config::TagType;
+    constexpr uint32_t L = Ops::kLookupLanes;
+    const uint64_t group = ((uint64_t(1) << L) - 1) << (__lane_id() & ~(L - 1));
+    const size_t total = n * L;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
+        const size_t i = t / L;
+        const uint32_t part = static_cast<uint32_t>(t % L);
+        const auto c = Policy::getCandidateBuckets(keys[i], v.numBuckets);
+        bool found = (__ballot(Ops::bucketPartHas(v, c.i1, part, c.fp)) & group) != 0;
+        if (!found) {
+            Tag fp2 = c.fp;
+            if constexpr (Policy::usesChoiceBit) fp2 = Ops::altWithNewFp(c.i1, c.fp, v.numBuckets).fp;
+            if (c.i2 != c.i1 || fp2 != c.fp) {
+                found = (__ballot(Ops::bucketPartHas(v, c.i2, part, fp2)) & group) != 0;
+            }
+        }
+        if (part == 0) output[i] = found;
+    }
+}
+
+}  // namespace detail
+
+/// containsKernel (CuckooFilter.cuh:1143-1155).  Buckets of 64 and 128 bytes
+/// are read by several lanes per key where FilterOps::groupedLookup says so;
+/// any grid covers all n keys.
 template <typename Config>
 __global__ __launch_bounds__(kMaxBlock) void containsKernel(const typename Config::KeyType* __restrict__ keys,
                                                             bool* __restrict__ output, size_t n,
                                                             DeviceView<Config> v) {
     using Ops = FilterOps<Config>;
+    if constexpr (Ops::kLookupLanes > 1) {
+        if (Ops::groupedLookup(v.numBuckets, blockDim.x)) {  // uniform across the grid
+            detail::containsGrouped<Config>(keys, output, n, v);
+            return;
+        }
+    }
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         output[i] = Ops::contains(v, keys[i]);
@@ -194,31 +237,32 @@ struct Workspace {
 
 namespace detail {
 
-/// Resident workgroups of `kernel` at `block` threads on device `dev`: the
-/// occupancy query runs once per (kernel, block, device) and thread, not per
-/// launch.
-inline uint32_t residentBlocks(const void* kernel, uint32_t block, int dev) {
+/// Resident workgroups of `kernel` at `block` threads and `lds` bytes of
+/// dynamic LDS on device `dev`: the occupancy query runs once per (kernel,
+/// block, LDS, device) and thread, not per launch.
+inline uint32_t residentBlocks(const void* kernel, uint32_t block, int dev, size_t lds = 0) {
     struct Entry {
         const void* kernel;
         uint32_t block;
         int dev;
+        size_t lds;
         uint32_t resident;
     };
     constexpr int kEntries = 64;
     static thread_local Entry cache[kEntries] = {};
     static thread_local int next = 0;
     for (const Entry& e : cache) {
-        if (e.kernel == kernel && e.block == block && e.dev == dev) return e.resident;
+        if (e.kernel == kernel && e.block == block && e.dev == dev && e.lds == lds) return e.resident;
     }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
     int perCu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, static_cast<int>(block), 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, static_cast<int>(block), lds) != hipSuccess ||
         perCu <= 0) {
         perCu = 1;
     }
     const uint32_t resident = static_cast<uint32_t>(perCu) * static_cast<uint32_t>(cus);
-    cache[next] = Entry{kernel, block, dev, resident};
+    cache[next] = Entry{kernel, block, dev, lds, resident};
     next = (next + 1) % kEntries;
     return resident;
 }

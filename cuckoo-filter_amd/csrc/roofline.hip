@@ -7,6 +7,8 @@
 //    multiply so the memory system is what is measured.  `dependent` makes
 //    each further block's address depend on the data of the previous one
 //    (the shape of a lookup's second probe).
+//  * rl_gather_coop: one random block per key read by 1-8 lanes together,
+//    each lane loading its part (fewer load instructions per lane and page).
 //  * rl_cas: per key, one relaxed device-scope 64-bit CAS at a random word
 //    (the write half of an insert).
 //  * rl_cas_xcd: the same CAS, but each workgroup reads the XCD it runs on
@@ -63,6 +65,27 @@ __global__ __launch_bounds__(1024) void gatherKernel(const V4* __restrict__ tabl
             acc ^= readBlock<GB, NT>(table + (h & blockMask) * (GB / 16));
         }
         out[i] = acc == 0x5A5A5A5Au;
+    }
+}
+
+/// rl_gather with `L` consecutive lanes per key, each reading its GB/L-byte
+/// part of the key's block (one translation per lane instead of GB/16), the
+/// parts combined with cross-lane shuffles.
+template <int GB, int L>
+__global__ __launch_bounds__(1024) void gatherCoopKernel(const V4* __restrict__ table, uint64_t blockMask,
+                                                         const uint64_t* __restrict__ keys, size_t n,
+                                                         uint8_t* __restrict__ out) {
+    static_assert(GB / L >= 16 && 64 % L == 0, "a part is at least one 16-byte load");
+    const size_t total = n * L;
+    const size_t stride = size_t(gridDim.x) * blockDim.x;  // a multiple of 64: a key's lanes move together
+    for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
+        const size_t i = t / L;
+        const uint32_t part = static_cast<uint32_t>(t % L);
+        const uint64_t h = mix(keys[i]);
+        unsigned int acc = readBlock<GB / L, false>(table + (h & blockMask) * (GB / 16) + part * (GB / L / 16));
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) acc ^= __shfl_xor(acc, off, 64);
+        if (part == 0) out[i] = acc == 0x5A5A5A5Au;
     }
 }
 
@@ -185,6 +208,36 @@ int rl_gather(const void* table, size_t table_bytes, int gather_bytes, const uin
         if (nontemporal) RL_GATHER(128, true); else RL_GATHER(128, false);
     }
 #undef RL_GATHER
+    return static_cast<int>(hipGetLastError());
+}
+
+// lanes: 1, 2, 4 or 8 lanes per key, with gather_bytes / lanes at least 16.
+int rl_gather_coop(const void* table, size_t table_bytes, int gather_bytes, int lanes, const uint64_t* keys,
+                   size_t n, uint8_t* out, int block, void* stream) {
+    if (!table || !keys || !out || (gather_bytes != 32 && gather_bytes != 64 && gather_bytes != 128) ||
+        (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) || gather_bytes / lanes < 16 || block < 64 ||
+        block > 1024 || block % 64 || table_bytes % gather_bytes || !pow2(table_bytes / gather_bytes))
+        return -1;
+    const uint64_t mask = table_bytes / gather_bytes - 1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const V4* t = static_cast<const V4*>(table);
+#define RL_COOP(GB, L)                                                                                          \
+    gatherCoopKernel<GB, L><<<gridFor((const void*)gatherCoopKernel<GB, L>, n * L, block), block, 0, s>>>(t, mask, \
+                                                                                                    keys, n, out)
+    const int key = gather_bytes * 16 + lanes;
+    switch (key) {
+        case 32 * 16 + 1: RL_COOP(32, 1); break;
+        case 32 * 16 + 2: RL_COOP(32, 2); break;
+        case 64 * 16 + 1: RL_COOP(64, 1); break;
+        case 64 * 16 + 2: RL_COOP(64, 2); break;
+        case 64 * 16 + 4: RL_COOP(64, 4); break;
+        case 128 * 16 + 1: RL_COOP(128, 1); break;
+        case 128 * 16 + 2: RL_COOP(128, 2); break;
+        case 128 * 16 + 4: RL_COOP(128, 4); break;
+        case 128 * 16 + 8: RL_COOP(128, 8); break;
+        default: return -1;
+    }
+#undef RL_COOP
     return static_cast<int>(hipGetLastError());
 }
 

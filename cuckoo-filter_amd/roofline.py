@@ -44,7 +44,9 @@ def _lib():
     L.rl_store_bytes.argtypes = [P, sz, P, sz, i, P]
     L.rl_copy.argtypes = [P, P, sz, i, P]
     L.rl_gather_region.argtypes = [P, sz, sz, P, sz, P, i, i, P]
-    for f in (L.rl_gather, L.rl_cas, L.rl_cas_xcd, L.rl_store_bytes, L.rl_copy, L.rl_gather_region):
+    L.rl_gather_coop.argtypes = [P, sz, i, i, P, sz, P, i, P]
+    for f in (L.rl_gather, L.rl_cas, L.rl_cas_xcd, L.rl_store_bytes, L.rl_copy, L.rl_gather_region,
+              L.rl_gather_coop):
         f.restype = i
     return L
 
@@ -115,6 +117,24 @@ def measure(table_bytes=512 << 20, n=1 << 26, reps=5, block=256, big_factor=4) -
                 assert rc == 0, rc
             t = _time(torch, cas_xcd, reps)
             res[f"cas8_xcd_{'wg' if wg else 'agent'}_{tag}_gops"] = round(n / t / 1e9, 2)
+        del tab
+        torch.cuda.empty_cache()
+
+    # address translation: past 2 GiB the random-gather rate falls with the
+    # load instructions each lane issues per request; the same blocks read by
+    # several lanes per key (one 16- or 32-byte part each) keep the rate
+    def coop(tab, tb, gb, lanes):
+        def run():
+            rc = L.rl_gather_coop(C.c_void_p(tab.data_ptr()), tb, gb, lanes, C.c_void_p(keys.data_ptr()), n,
+                                  C.c_void_p(out.data_ptr()), block, s)
+            assert rc == 0, rc
+        return n / _time(torch, run, reps) / 1e9
+
+    for tb in (table_bytes, 8 * table_bytes):
+        tab = torch.randint(0, 2**31 - 1, (tb // 4,), dtype=torch.int32, device=dev)
+        tag = f"{tb >> 20}MiB"
+        for gb, lanes in ((32, 1), (64, 1), (64, 2), (64, 4), (128, 1), (128, 2), (128, 4)):
+            res[f"gather{gb}_lanes{lanes}_{tag}_greq"] = round(coop(tab, tb, gb, lanes), 1)
         del tab
         torch.cuda.empty_cache()
 
