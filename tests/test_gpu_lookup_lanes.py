@@ -12,11 +12,16 @@ result must be the reference's lookup (include/CuckooFilter.cuh:1088-1098,
     (4, 64, 256, 1024) and ones that do not (100, 250), with a ragged number
     of keys;
   * on 4 GiB tables (2^30 slots, 32-bit tags; buckets of 32 slots in u64
-    words, of 16 in u64 and u32 words) the GPU filled to 95%: 2^24 lookups, half of them inserted keys, against
-    the oracle on the exported image.
+    words, of 16 in u64 and u32 words) the GPU filled to 95%: 2^24 lookups,
+    half of them inserted keys, against the oracle on the exported image;
+    then 2^24 deletes (the binned path with 64-bit partition entries) through
+    the delete-parity check of tests/delete_parity.py.
 """
 import numpy as np
 import pytest
+
+from conftest import assert_table_count
+from delete_parity import check_delete_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -77,18 +82,27 @@ def test_grouped_lookup_on_4gib_table(cf, oracle_mod, bucket, word):
     occ = f.insertMany(keys[:n])
     print(f"4 GiB table: {occ} of {n} placed", flush=True)
     assert occ == n
+    assert_table_count(f, occ, n)
     q = torch.cat([keys[:nq], keys[n:]])
     got = f.containsMany(q).cpu().numpy()
     qh = q.cpu().numpy().view(np.uint64)
-    del keys
+    nd = 1 << 24  # a binned delete on 2^25 buckets (its 64-bit-entry partition)
+    dkeys = keys[:nd].clone()
+    del keys, q
     torch.cuda.empty_cache()
     image = f.exportTable()
     of = O.Filter(oc, cap)
     of.load_table(image, occ)
-    del image
     want = of.contains_many(qh, threads=16)
+    del of
     mism = np.nonzero(got != want)[0]
     assert mism.size == 0, f"{mism.size} lookups differ, first at {mism[0]}"
     assert got[:nq].all()
     print(f"2^24 lookups bit-exact on the 4 GiB image ({int(got[nq:].sum())} false positives)", flush=True)
+    out = torch.zeros(nd, dtype=torch.bool, device=dkeys.device)
+    rem = f.deleteMany(dkeys, out=out)
+    st = check_delete_parity(O, oc, cap, image, occ, dkeys.cpu().numpy().view(np.uint64), out.cpu().numpy(),
+                             f.exportTable(), rem)
+    print(f"2^24 deletes on the 4 GiB image: {st}", flush=True)
+    assert st["successes"] == nd
     f.close()
