@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kBinBlock) void binInsertKernel(const E* __restrict
             bool placed = true;
             if (valid) {
                 Word* bw = slice + static_cast<size_t>(local) * WC;
-                placed = Ops::tryInsertSnap(bw, detail::loadBucket<Word, WC>(bw), fp);
+                placed = Ops::tryInsertWordsLds(bw, fp);
                 ok += placed;
             }
             appendOverflow(!placed, ((b0 + local) << 32) | static_cast<uint64_t>(fp), ovBuf, &ovCount,
@@ -570,7 +570,7 @@ __global__ __launch_bounds__(kBinBlock) void binDeleteKernel(const E* __restrict
             bool removed = true;
             if (valid) {
                 Word* bw = slice + static_cast<size_t>(local) * WC;
-                removed = Ops::tryRemoveSnap(bw, detail::loadBucket<Word, WC>(bw), fp);
+                removed = Ops::tryRemoveWordsLds(bw, fp);
                 ok += removed;
             }
             appendOverflow(!removed, ((b0 + local) << 32) | static_cast<uint64_t>(fp), ovBuf, &ovCount,

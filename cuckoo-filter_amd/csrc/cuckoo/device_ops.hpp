@@ -268,7 +268,7 @@ struct FilterOps {
     /// The first step of tryInsertAtBucket on a bucket in LDS: one attempt at
     /// the tag's start word, where the reference's scan starts
     /// (CuckooFilter.cuh:866-872).  False if that word is full or the CAS
-    /// lost; the caller then continues with the whole bucket (tryInsertSnap).
+    /// lost; the caller then continues with the whole scan (tryInsertWordsLds).
     static __device__ __forceinline__ bool tryStartWordLds(Word* bw, Tag tag) {
         const uint32_t start = startWord(tag);
         const Word cur = bw[start];
@@ -276,6 +276,50 @@ struct FilterOps {
         if (zm == 0) return false;
         const uint32_t lane = lowestBit(zm) / BITS;
         return detail::casRelaxed(bw + start, cur, replaceTag(cur, lane, tag)) == cur;
+    }
+
+    /// tryInsertAtBucket on a bucket in LDS as the reference scans it
+    /// (CuckooFilter.cuh:859-904): word by word from the tag's start word,
+    /// reading a word only when the ones before it are full.  In LDS the
+    /// reads cost bank cycles, not round trips, so the early exit beats one
+    /// whole-bucket read.
+    static __device__ __forceinline__ bool tryInsertWordsLds(Word* bw, Tag tag) {
+        const uint32_t start = startWord(tag);
+#pragma unroll
+        for (uint32_t i = 0; i < WC; ++i) {
+            const uint32_t w = (start + i) & (WC - 1);
+            Word cur = bw[w];
+            for (;;) {
+                const Word zm = getZeroMask<Tag, Word>(cur);
+                if (zm == 0) break;
+                const uint32_t lane = lowestBit(zm) / BITS;
+                const Word old = detail::casRelaxed(bw + w, cur, replaceTag(cur, lane, tag));
+                if (old == cur) return true;
+                cur = old;
+            }
+        }
+        return false;
+    }
+
+    /// tryRemoveAtBucket on a bucket in LDS, word by word from the start word
+    /// (CuckooFilter.cuh:805-848); a tag is most often found in its start word.
+    static __device__ __forceinline__ bool tryRemoveWordsLds(Word* bw, Tag tag) {
+        const uint32_t start = startWord(tag);
+        const Word rep = replicateTag<Tag, Word>(tag);
+#pragma unroll
+        for (uint32_t i = 0; i < WC; ++i) {
+            const uint32_t w = (start + i) & (WC - 1);
+            Word cur = bw[w];
+            for (;;) {
+                const Word mm = getZeroMask<Tag, Word>(cur ^ rep);
+                if (mm == 0) break;
+                const uint32_t lane = lowestBit(mm) / BITS;
+                const Word old = detail::casRelaxed(bw + w, cur, replaceTag(cur, lane, EMPTY));
+                if (old == cur) return true;
+                cur = old;
+            }
+        }
+        return false;
     }
 
     /// tryRemoveAtBucket on a snapshot (CuckooFilter.cuh:805-848).
