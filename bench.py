@@ -268,7 +268,7 @@ def batched_section(cf, torch, workloads, dev, n_filters=256, n_batches=8192, se
 
 
 def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(0.5, 0.7, 1.0),
-                    batched_mkeys_s=None, capped_levels=(), host_burst_run=True):
+                    batched_mkeys_s=None, capped_levels=(), host_burst_run=True, orderings=("arrival", "phased")):
     """Serving workload: canonical 31-mers of a 2^28-base ACGT sequence (10%
     of it in duplicated regions), 2-bit packed on the device, in one 2^30-slot
     filter, plus 64 tenant filters of 2^20..2^24 slots (log-uniform, filled to
@@ -281,7 +281,13 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
     HBM.  The trace first runs as one burst (every request queued at t = 0)
     to find the saturated throughput, then open-loop with Poisson arrivals at
     each fraction of it in `loads`; every run starts from the same filter
-    states.  Latency = completion - scheduled arrival."""
+    states.  Latency = completion - scheduled arrival.  The runs repeat for
+    each of `orderings`: "arrival" (default server options: results as if
+    one by one in arrival order; its figures are the top-level keys) and
+    "phased" (a window's requests concurrent: per filter its inserts, then
+    its lookups and deletes together; windows formed once the one before has
+    reached its last level and capped at 2^24 keys; under the "phased"
+    key)."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -450,8 +456,15 @@ def serving_section(cf, torch, workloads, dev, n_requests=20000, seed=5, loads=(
             offered = 0.7 * batched_mkeys_s
             into["load_70pct_of_batched"] = dict(run(srv, poisson(offered)), offered_mkeys_s=round(offered, 1))
 
-    with cf.Server() as srv:
-        sweep(srv, out)
+    if "arrival" in orderings:
+        with cf.Server() as srv:
+            sweep(srv, out)
+    if "phased" in orderings:
+        opts = dict(ordering="phased", form_late=True, max_window_keys=1 << 24)
+        part = {"options": {k: v for k, v in opts.items()}}
+        with cf.Server(**opts) as srv:
+            sweep(srv, part)
+        out["phased"] = part
     for cap in capped_levels:
         # windows capped at `cap` levels: shorter windows, so a request waits
         # for less work ahead of it

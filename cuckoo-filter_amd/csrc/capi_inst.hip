@@ -101,9 +101,10 @@ struct Entry {
         return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->execute(ops, n, s, occupiedOut);
     }
     static hipError_t batchLaunch(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t s,
-                                  bool useGraph, uint32_t chunkKeys) {
+                                  bool useGraph, uint32_t chunkKeys, bool phased, hipEvent_t beforeLast) {
         if (*state == nullptr) *state = new cuckoo::BatchExecutor<Cfg>();
-        return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->launch(ops, n, s, useGraph, chunkKeys);
+        return static_cast<cuckoo::BatchExecutor<Cfg>*>(*state)->launch(ops, n, s, useGraph, chunkKeys, phased,
+                                                                          beforeLast);
     }
     static const unsigned long long* batchResults(const void* state) {
         return static_cast<const cuckoo::BatchExecutor<Cfg>*>(state)->results();

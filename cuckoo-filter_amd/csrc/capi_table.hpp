@@ -50,7 +50,7 @@ struct Ops {
     // asynchronous form: launches on the stream; batchResults gives the
     // post-op occupancies once the stream has passed the launch
     hipError_t (*batchLaunch)(void** state, const cuckoo::BatchOpDesc* ops, size_t n, hipStream_t, bool useGraph,
-                              uint32_t chunkKeys);
+                              uint32_t chunkKeys, bool phased, hipEvent_t beforeLast);
     const unsigned long long* (*batchResults)(const void* state);
     uint32_t (*batchLevels)(const void* state);
     void (*batchRelease)(void* state);

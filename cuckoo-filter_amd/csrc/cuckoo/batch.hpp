@@ -8,7 +8,13 @@
 // commute with each other but not with inserts or deletes.  Every batch gets
 // a *level*: a lookup runs one level after the last earlier insert/delete on
 // its filter, an insert/delete one level after every earlier batch on its
-// filter.  The levels run in order, each as one grid-stride kernel over
+// filter.  A phased window (the serving loop's phased ordering) instead
+// treats its batches as concurrent: each filter's inserts run as one level
+// (one concurrent insert), then its lookups and deletes together in the next.
+// A delete never moves a tag, so a lookup running beside deletes differs
+// from one before them only for keys those deletes remove; inserts move tags
+// (evictions) and so stay in a level of their own.  The levels run in order,
+// each as one grid-stride kernel over
 // fixed-size key chunks of all of its batches' keys (512 keys, two per thread,
 // for throughput; the serving loop uses 256, one per thread, for latency).
 // In a level with an insert or delete, the last workgroup to finish records
@@ -279,12 +285,16 @@ class BatchExecutor {
     /// chunk count (cheapest when every window differs, as in a server).
     /// chunkKeys is the keys per workgroup step: kBatchChunk (two per
     /// thread) or kBatchChunkLatency (one per thread); graphs are kept for
-    /// kBatchChunk only, other windows run level by level.
+    /// kBatchChunk only, other windows run level by level.  phased plans the
+    /// window in two phases per filter (its inserts, then its lookups and
+    /// deletes) instead of in window order; every op then reports its
+    /// filter's occupancy after its phase.  beforeLast (optional, level-by-level launches) is
+    /// recorded on the stream just before the window's last level.
     hipError_t launch(const BatchOpDesc* ops, size_t count, hipStream_t stream, bool useGraph = true,
-                      uint32_t chunkKeys = kBatchChunk) {
+                      uint32_t chunkKeys = kBatchChunk, bool phased = false, hipEvent_t beforeLast = nullptr) {
         if (count == 0) return hipSuccess;
         if (chunkKeys != kBatchChunk && chunkKeys != kBatchChunkLatency) return hipErrorInvalidValue;
-        const uint32_t numLevels = plan(ops, count, chunkKeys);
+        const uint32_t numLevels = plan(ops, count, chunkKeys, phased);
         if (stageError_ != hipSuccess) return stageError_;
         hipError_t e;
         bool moved = false;
@@ -301,7 +311,7 @@ class BatchExecutor {
             if ((e = hipGraphLaunch(graphs_[numLevels], stream)) != hipSuccess) return e;
         } else {
             // level by level, each with a grid no larger than its chunk count
-            const uint32_t grid = levelGrid();
+            const uint32_t grid = levelGrid(chunkKeys, phased);
             LevelArgs la{};
             la.ops = reinterpret_cast<const BatchOpDesc*>(win + hdr_.opsOff);
             la.levelOps = reinterpret_cast<const uint32_t*>(win + hdr_.levelOpsOff);
@@ -318,6 +328,9 @@ class BatchExecutor {
                 la.r0 = l > 0 && levelMode_[l - 1] == kMutatesDeferred ? opStart_[l - 1] : la.p0;
                 const uint32_t chunks = la.g1 - la.g0;
                 const uint32_t g = chunks == 0 ? 1u : (chunks < grid ? chunks : grid);
+                if (beforeLast != nullptr && l + 1 == numLevels &&
+                    (e = hipEventRecord(beforeLast, stream)) != hipSuccess)
+                    return e;
                 if (chunkKeys == kBatchChunk) {
                     batchLevelKernel<Config, kBatchChunk><<<g, kBatchBlock, 0, stream>>>(win, l, results, la);
                 } else {
@@ -354,11 +367,28 @@ class BatchExecutor {
 
     /// Writes the window (header, descriptors, chunks, level tables) into the
     /// pinned staging buffer; returns the level count.
-    uint32_t plan(const BatchOpDesc* ops, size_t count, uint32_t chunk) {
+    uint32_t plan(const BatchOpDesc* ops, size_t count, uint32_t chunk, bool phased) {
         seen_.clear();
         level_.resize(count);
         uint32_t numLevels = 0;
-        for (size_t i = 0; i < count; ++i) {
+        if (phased) {
+            // level 1 for a lookup or delete on a filter that has inserts in
+            // the window, else level 0
+            kinds_.clear();
+            filterOf_.resize(count);
+            for (size_t i = 0; i < count; ++i) {
+                auto [it, fresh] = seen_.try_emplace(ops[i].occupied, static_cast<uint32_t>(kinds_.size()));
+                if (fresh) kinds_.push_back(0);
+                filterOf_[i] = it->second;
+                if (ops[i].kind == static_cast<uint32_t>(BatchOpKind::Insert)) kinds_[it->second] = 1;
+            }
+            for (size_t i = 0; i < count; ++i) {
+                const bool insert = ops[i].kind == static_cast<uint32_t>(BatchOpKind::Insert);
+                level_[i] = !insert && kinds_[filterOf_[i]] != 0 ? 1u : 0u;
+                numLevels = std::max(numLevels, level_[i] + 1);
+            }
+        }
+        for (size_t i = 0; i < count && !phased; ++i) {
             auto [it, fresh] = seen_.try_emplace(ops[i].occupied, static_cast<uint32_t>(state_.size()));
             if (fresh) state_.push_back(FilterLevelsPod{-1, -1});
             FilterLevelsPod& st = state_[it->second];
@@ -473,8 +503,24 @@ class BatchExecutor {
         return e;
     }
 
-    static uint32_t levelGrid() {
-        return detail::gridFor(batchLevelKernel<Config, kBatchChunk>, static_cast<size_t>(1) << 30, kBatchBlock);
+    /// Level grid: the resident workgroups of the two-keys-per-thread kernel
+    /// (graph windows, and the serving loop's arrival-ordered windows: 6 per
+    /// CU), and three workgroups per CU for phased windows.  Measured on the
+    /// serving workload (2^30-slot k-mer filter + 64 tenants, phased windows
+    /// of 2^22 keys): 512 / 768 / 1024 / 1536 / 2048 workgroups gave 15.3 /
+    /// 15.7 / 15.1 / 13.3 / 11.9 Gkeys/s (fewer device-scope CAS in flight
+    /// queue less at the memory side).  Arrival-ordered windows hold many
+    /// small levels, whose latency rises when fewer workgroups share a level
+    /// (p99 at half load 0.25-0.39 ms at 1536, 0.75 ms at 768); graph windows
+    /// of the batched workload ran fastest at full residency (23.8 Gkeys/s at
+    /// 1536 against 20.0 at 768).
+    static uint32_t levelGrid(uint32_t chunkKeys, bool phased) {
+        constexpr size_t kMany = static_cast<size_t>(1) << 30;
+        const uint32_t resident = detail::gridFor(batchLevelKernel<Config, kBatchChunk>, kMany, kBatchBlock);
+        if (chunkKeys == kBatchChunk || !phased) return resident;
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        return std::min(resident, 3u * detail::computeUnits(dev));
     }
 
     void dropGraphs() {
@@ -491,7 +537,7 @@ class BatchExecutor {
             return e;
         auto* win = static_cast<unsigned char*>(dWin_);
         auto* results = static_cast<unsigned long long*>(dResults_);
-        const uint32_t grid = levelGrid();
+        const uint32_t grid = levelGrid(kBatchChunk, false);
         if ((e = hipStreamBeginCapture(captureStream_, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
         for (uint32_t l = 0; l < numLevels; ++l) {
             batchLevelKernel<Config, kBatchChunk><<<grid, kBatchBlock, 0, captureStream_>>>(win, l, results,
@@ -524,6 +570,7 @@ class BatchExecutor {
     std::unordered_map<const void*, uint32_t> seen_;
     std::vector<FilterLevelsPod> state_;
     std::vector<uint32_t> level_, opStart_, levelOps_, chunkCum_, cursor_, levelMode_;
+    std::vector<uint32_t> kinds_, filterOf_;  // phased planning: per filter, has inserts; each op's filter
     WindowHeader hdr_{};  // the last planned window's header (its offsets into the device window)
     // graphs by level count
     hipStream_t captureStream_ = nullptr;

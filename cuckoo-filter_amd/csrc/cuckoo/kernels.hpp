@@ -267,6 +267,17 @@ inline uint32_t residentBlocks(const void* kernel, uint32_t block, int dev, size
     return resident;
 }
 
+/// Compute units of device `dev` (queried once per device and thread).
+inline uint32_t computeUnits(int dev) {
+    constexpr int kDevices = 64;
+    static thread_local int cached[kDevices] = {};
+    if (dev >= 0 && dev < kDevices && cached[dev] > 0) return static_cast<uint32_t>(cached[dev]);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
+    if (dev >= 0 && dev < kDevices) cached[dev] = cus;
+    return static_cast<uint32_t>(cus);
+}
+
 /// Grid for a grid-stride kernel: enough blocks for n keys, capped at the
 /// number of workgroups the device keeps resident.
 template <typename Kernel>

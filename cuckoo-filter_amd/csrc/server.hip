@@ -6,7 +6,9 @@
 // on the server's stream; the completer thread waits for windows in launch
 // order, copies host-memory results out and publishes per-request results.
 // One stream runs every window, so windows execute in launch order and the
-// batch executor's per-window ordering gives arrival order per filter.
+// batch executor's per-window ordering gives arrival order per filter (or,
+// under the phased ordering, the window's two phases: its inserts, then its
+// lookups and deletes together).
 // Window slots (two by default, window_slots in the options): one window runs
 // on the device while the next ones are formed, staged and queued behind it.
 // A third slot lets the host copy of one window overlap the upload of the one
@@ -166,6 +168,7 @@ struct Slot {
     void* hArena = nullptr;  // pinned host staging (same layout)
     hipEvent_t done = nullptr;
     hipEvent_t uploaded = nullptr;  // the window's keys are on the device (copy stream)
+    hipEvent_t lastLevel = nullptr;  // the window has reached its last level
     // the window in this slot
     std::vector<Pending> reqs;
     std::vector<cuckoo::BatchOpDesc> descs;
@@ -211,7 +214,13 @@ struct cf_server {
     std::thread completer;
 
     uint32_t maxLevels = 0;  // levels per window (0 in the options: unlimited)
-    std::unordered_map<const cf_filter*, std::pair<int32_t, int32_t>> depth;  // launcher's planning scratch
+    bool phased = false;     // CF_ORDER_PHASED
+    bool formLate = false;   // form a window once the one before it has reached its last level
+    int lastLaunched = -1;   // slot of the window launched last (launcher thread only)
+    // launcher's planning scratch: (last mutation level, last level) per filter;
+    // phased: the phases present per filter, as a bit mask
+    std::unordered_map<const cf_filter*, std::pair<int32_t, int32_t>> depth;
+    std::unordered_map<const cf_filter*, uint32_t> kindsSeen;
 
     void launcherLoop();
     void completerLoop();
@@ -276,7 +285,7 @@ int cf_server::launchWindow(Slot& sl) {
     // keys run side by side (a level lasts as long as its slowest workgroup).
     if (e == hipSuccess) {
         e = sl.ops->batchLaunch(st, sl.descs.data(), sl.descs.size(), stream, false,
-                                cuckoo::kBatchChunkLatency);
+                                cuckoo::kBatchChunkLatency, phased, sl.lastLevel);
     }
     sl.state = *st;
     if (e == hipSuccess && outBytes > keyBytes)
@@ -298,6 +307,20 @@ void cf_server::launcherLoop() {
     for (;;) {
         cvWork.wait(lk, [&] { return (stopping && queue.empty()) || (!queue.empty() && !freeSlots.empty()); });
         if (queue.empty()) break;  // stopping and drained
+        if (formLate && lastLaunched >= 0) {
+            // Wait until the window launched last has reached its last level,
+            // so that the requests arriving until then join the next window
+            // instead of one formed behind it.  Only this thread reuses a
+            // slot, so the event stays the one that window recorded.
+            const hipEvent_t ev = slots[lastLaunched].lastLevel;
+            lk.unlock();
+            uint32_t spins = 0;
+            while (hipEventQuery(ev) == hipErrorNotReady) {
+                if (++spins > 2000) std::this_thread::sleep_for(std::chrono::microseconds(10));
+                else std::this_thread::yield();
+            }
+            lk.lock();
+        }
         const int si = freeSlots.back();
         freeSlots.pop_back();
         Slot& sl = slots[si];
@@ -307,8 +330,10 @@ void cf_server::launcherLoop() {
         size_t stage = 0;
         // the level each request would take in the window (the batch planner's
         // rule: a lookup one level after its filter's last insert / delete, an
-        // insert / delete one level after its filter's last request)
+        // insert / delete one level after its filter's last request; phased:
+        // one level for the filter's inserts, one for its lookups and deletes)
         depth.clear();
+        kindsSeen.clear();
         while (!queue.empty() && sl.reqs.size() < opt.max_window_requests) {
             const Pending& p = queue.front();
             if (p.req.filter->ops != sl.ops) break;  // one configuration per window
@@ -318,12 +343,20 @@ void cf_server::launcherLoop() {
             (void)fresh;
             auto& [lastMutation, maxLevel] = it->second;
             const bool lookup = p.req.op == CF_OP_LOOKUP;
-            const int32_t level = lookup ? lastMutation + 1 : maxLevel + 1;
-            if (!sl.reqs.empty() && static_cast<uint32_t>(level) >= maxLevels) break;
-            if (lookup) {
-                maxLevel = std::max(maxLevel, level);
+            if (phased) {
+                // bit 0: inserts; bit 1: lookups or deletes
+                uint32_t& seen = kindsSeen[p.req.filter];
+                const uint32_t kinds = seen | (p.req.op == CF_OP_INSERT ? 1u : 2u);
+                if (!sl.reqs.empty() && static_cast<uint32_t>(__builtin_popcount(kinds)) > maxLevels) break;
+                seen = kinds;
             } else {
-                lastMutation = maxLevel = level;
+                const int32_t level = lookup ? lastMutation + 1 : maxLevel + 1;
+                if (!sl.reqs.empty() && static_cast<uint32_t>(level) >= maxLevels) break;
+                if (lookup) {
+                    maxLevel = std::max(maxLevel, level);
+                } else {
+                    lastMutation = maxLevel = level;
+                }
             }
             keys += p.req.n;
             stage += p.stageKeys + p.stageOut;
@@ -331,6 +364,7 @@ void cf_server::launcherLoop() {
             queue.pop_front();
         }
         sl.id = nextWindow++;
+        lastLaunched = si;
         lk.unlock();
         sl.startNs = nowNs();
         sl.status = launchWindow(sl);
@@ -470,6 +504,7 @@ void freeServer(cf_server* s) {
         if (sl.hArena != nullptr) (void)hipHostFree(sl.hArena);
         if (sl.done != nullptr) (void)hipEventDestroy(sl.done);
         if (sl.uploaded != nullptr) (void)hipEventDestroy(sl.uploaded);
+        if (sl.lastLevel != nullptr) (void)hipEventDestroy(sl.lastLevel);
     }
     if (s->copyStream != nullptr) (void)hipStreamDestroy(s->copyStream);
     if (s->stream != nullptr) (void)hipStreamDestroy(s->stream);
@@ -492,6 +527,8 @@ int cf_server_options_init(cf_server_options* o) {
     o->staging_bytes = 0;
     o->window_slots = 2;
     o->max_window_levels = 0;
+    o->ordering = CF_ORDER_ARRIVAL;
+    o->form_late = 0;
     return CF_OK;
 }
 
@@ -508,10 +545,14 @@ int cf_server_create(const cf_server_options* o, cf_server** out) {
         return fail(CF_ERR_INVALID, "window limits must be positive");
     if (opt.window_slots < 2 || opt.window_slots > kMaxSlots)
         return fail(CF_ERR_INVALID, "window_slots must be 2..%d (got %u)", kMaxSlots, opt.window_slots);
+    if (opt.ordering != CF_ORDER_ARRIVAL && opt.ordering != CF_ORDER_PHASED)
+        return fail(CF_ERR_INVALID, "unknown ordering %u", opt.ordering);
     auto* s = new (std::nothrow) cf_server();
     if (s == nullptr) return fail(CF_ERR_ALLOC, "host allocation failed");
     s->opt = opt;
     s->maxLevels = opt.max_window_levels != 0 ? opt.max_window_levels : UINT32_MAX;
+    s->phased = opt.ordering == CF_ORDER_PHASED;
+    s->formLate = opt.form_late != 0;
     DeviceGuard g(opt.device);
     hipError_t e = hipSuccess;
     size_t staging = opt.staging_bytes;
@@ -535,6 +576,7 @@ int cf_server_create(const cf_server_options* o, cf_server** out) {
         if (e == hipSuccess) e = hipHostMalloc(&sl.hArena, s->arenaBytes, hipHostMallocDefault);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.uploaded, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.lastLevel, hipEventDisableTiming);
         s->freeSlots.push_back(s->numSlots - 1 - i);
     }
     if (e != hipSuccess) {

@@ -47,6 +47,7 @@ class cf_batch_op(C.Structure):
 
 
 CF_REQUEST_HOST_MEMORY = 1
+CF_ORDER_ARRIVAL, CF_ORDER_PHASED = 0, 1
 
 # cf_request / cf_request_result / cf_server_options / cf_server_stats as numpy
 # records (same layout as the C structs), so request lists need no per-item
@@ -56,11 +57,12 @@ REQUEST_DTYPE = np.dtype([("filter", "u8"), ("op", "u4"), ("flags", "u4"), ("key
 RESULT_DTYPE = np.dtype([("status", "i4"), ("window_levels", "u4"), ("window", "u8"), ("occupied", "u8"),
                          ("arrival_ns", "u8"), ("start_ns", "u8"), ("done_ns", "u8")])
 SERVER_OPTIONS_DTYPE = np.dtype([("device", "i4"), ("max_window_requests", "u4"), ("max_window_keys", "u8"),
-                                 ("staging_bytes", "u8"), ("window_slots", "u4"), ("max_window_levels", "u4")])
+                                 ("staging_bytes", "u8"), ("window_slots", "u4"), ("max_window_levels", "u4"),
+                                 ("ordering", "u4"), ("form_late", "u4")])
 SERVER_STATS_DTYPE = np.dtype([(k, "u8") for k in ("windows", "requests", "keys", "levels", "staging_bytes",
                                                     "max_queue")])
 assert REQUEST_DTYPE.itemsize == 40 and RESULT_DTYPE.itemsize == 48
-assert SERVER_OPTIONS_DTYPE.itemsize == 32 and SERVER_STATS_DTYPE.itemsize == 48
+assert SERVER_OPTIONS_DTYPE.itemsize == 40 and SERVER_STATS_DTYPE.itemsize == 48
 
 
 class cf_config(C.Structure):
@@ -502,12 +504,16 @@ class Server:
 
     A request is (filter, op, keys, out, host): keys / out are device tensors,
     or numpy arrays with host=True (staged through the server's device arena).
-    Results equal running the requests one by one in arrival order.
+    Results equal running the requests one by one in arrival order; with
+    ordering="phased" a window's requests are concurrent and run per filter as
+    its inserts (together), then its lookups and deletes (together).
     window_slots (2..8, default 2) is how many windows are formed ahead of the
-    running one; 3 lets host-memory staging overlap the uploads."""
+    running one; 3 lets host-memory staging overlap the uploads.  With
+    form_late a window is formed only once the one before it has reached its
+    last level, so that requests arriving meanwhile join it."""
 
     def __init__(self, device=None, max_window_requests=None, max_window_keys=None, staging_bytes=None,
-                 window_slots=None, max_window_levels=None):
+                 window_slots=None, max_window_levels=None, ordering="arrival", form_late=False):
         o = np.zeros(1, SERVER_OPTIONS_DTYPE)
         _check(lib.cf_server_options_init(o.ctypes.data))
         if device is not None:
@@ -522,6 +528,8 @@ class Server:
             o["window_slots"] = window_slots
         if max_window_levels is not None:
             o["max_window_levels"] = max_window_levels
+        o["ordering"] = {"arrival": CF_ORDER_ARRIVAL, "phased": CF_ORDER_PHASED}[ordering]
+        o["form_late"] = int(bool(form_late))
         h = C.c_void_p()
         _check(lib.cf_server_create(o.ctypes.data, C.byref(h)))
         self._h = h

@@ -228,7 +228,8 @@ int cf_run_plain_calls(const cf_batch_op* ops, size_t count, void* stream, uint6
  * stream; a completer thread retires windows in order and publishes the
  * results.  Two window slots keep one window forming while the other runs.
  * Results equal running the requests one by one, in arrival order, through
- * the plain calls.  Requests with CF_REQUEST_HOST_MEMORY pass host buffers:
+ * the plain calls (or, under CF_ORDER_PHASED, in the phase order given
+ * below).  Requests with CF_REQUEST_HOST_MEMORY pass host buffers:
  * their keys and results travel through a device staging arena, and such a
  * request is admitted to a window only when its bytes fit in the arena
  * (admission waits, in arrival order, for running windows to free space).
@@ -248,8 +249,30 @@ typedef struct cf_server_options {
     uint32_t max_window_levels;   /* 0 = unlimited (default); else a window stops taking requests
                                      before one that would need more levels than this (a level
                                      per insert / delete on one filter, and one per run of
-                                     lookups between them), which bounds a window's duration */
+                                     lookups between them; phased: at most two), which bounds
+                                     a window's duration */
+    uint32_t ordering;            /* CF_ORDER_ARRIVAL (default) or CF_ORDER_PHASED */
+    uint32_t form_late;           /* 0 (default): the next window is formed as soon as a slot is
+                                     free; 1: only once the window launched before it has reached
+                                     its last level, so that requests arriving until then join it
+                                     (lower latency below saturation, a short gap between windows) */
 } cf_server_options;
+
+/* Orderings of the requests on one filter.
+ * CF_ORDER_ARRIVAL: results equal running the requests one by one, in
+ *   arrival order, through the plain calls.
+ * CF_ORDER_PHASED: the requests of one window are concurrent; windows run in
+ *   arrival order.  On each filter the window's inserts run first, together
+ *   as one concurrent insert, then its lookups and deletes together.  A
+ *   lookup therefore sees every insert that arrived before it (and may see
+ *   inserts of its window that arrived after it); for a key that a delete of
+ *   its own window removes it may report either answer, and every other key
+ *   gets the answer it gets after the window's inserts.  Deletes never move
+ *   a tag, which is why they may run beside lookups; inserts move tags and
+ *   run alone.  Every request's occupied count is the filter's after its
+ *   phase.  A window takes at most two levels. */
+#define CF_ORDER_ARRIVAL 0u
+#define CF_ORDER_PHASED 1u
 
 typedef struct cf_request {
     cf_filter* filter;

@@ -136,11 +136,12 @@ def test_struct_layouts_match_header(cfm, tmp_path):
 
 
 @pytest.mark.parametrize("field,value", [("window_slots", 1), ("window_slots", 9), ("max_window_requests", 0),
-                                         ("max_window_keys", 0)])
+                                         ("max_window_keys", 0), ("ordering", 2)])
 def test_server_options_rejected_before_any_device_call(cfm, field, value):
     o = np.zeros(1, cfm.SERVER_OPTIONS_DTYPE)
     cfm.lib.cf_server_options_init(o.ctypes.data)
     assert int(o["window_slots"][0]) == 2 and int(o["max_window_levels"][0]) == 0
+    assert int(o["ordering"][0]) == cfm.CF_ORDER_ARRIVAL
     o[field] = value
     h = ctypes.c_void_p()
     rc = cfm.lib.cf_server_create(o.ctypes.data, ctypes.byref(h))

@@ -121,8 +121,9 @@ def test_server_keeps_arrival_order_per_filter(cf, oracle_mod, max_levels):
     assert f.occupiedSlots() == 0 and g.occupiedSlots() == 30000
 
 
-@pytest.mark.parametrize("staging,max_levels", [(None, None), (1 << 20, None), (None, 3)])
-def test_server_mixed_stream_matches_plain(cf, oracle_mod, staging, max_levels):
+@pytest.mark.parametrize("staging,max_levels,late", [(None, None, False), (1 << 20, None, False), (None, 3, False),
+                                                     (None, None, True)])
+def test_server_mixed_stream_matches_plain(cf, oracle_mod, staging, max_levels, late):
     import torch
 
     O = oracle_mod
@@ -186,7 +187,7 @@ def test_server_mixed_stream_matches_plain(cf, oracle_mod, staging, max_levels):
             out = torch.zeros(max(keys.size, 1), dtype=torch.bool, device="cuda") if kind else None
         reqs.append((served[i], kind, k, out, host))
     arrivals = np.cumsum(rng.exponential(1e-4, size=len(reqs)))
-    with cf.Server(staging_bytes=staging, max_window_levels=max_levels) as srv:
+    with cf.Server(staging_bytes=staging, max_window_levels=max_levels, form_late=late) as srv:
         res = srv.replay(srv.make_requests(reqs), arrivals)
         st = srv.stats()
     assert (res["status"] == 0).all()
@@ -208,6 +209,156 @@ def test_server_mixed_stream_matches_plain(cf, oracle_mod, staging, max_levels):
         assert_table_count(b, b.occupiedSlots(), b.capacity())
     if staging:
         assert st["staging_bytes"] <= staging
+
+
+def _phase_model(plan, windows, start_counts, start_present):
+    """Expected results of a phased run, given the window each request ran in:
+    per window and filter, the inserts (all of them), then the lookups and
+    deletes together.  plan[j] = (filter index, kind, keys); returns per
+    request the expected occupancy (its filter's after its phase) and, for
+    lookups, the keys that must be found and the keys a delete of the same
+    window removes (either answer is allowed for those)."""
+    count = list(start_counts)
+    present = [set(p) for p in start_present]
+    occ = [None] * len(plan)
+    must = [None] * len(plan)
+    maybe = [None] * len(plan)
+    for w in np.unique(windows):
+        idx = [j for j in range(len(plan)) if windows[j] == w]
+        for j in idx:  # phase 1: inserts (cf.OP_INSERT == 0)
+            i, k, keys = plan[j]
+            if k == 0:
+                present[i].update(map(int, keys))
+                count[i] += keys.size
+        for j in idx:
+            if plan[j][1] == 0:
+                occ[j] = count[plan[j][0]]
+        gone = {}  # phase 2: keys deleted in this window, per filter (cf.OP_DELETE == 2)
+        for j in idx:
+            i, k, keys = plan[j]
+            if k == 2:
+                gone.setdefault(i, set()).update(map(int, keys))
+        for j in idx:
+            i, k, keys = plan[j]
+            if k == 1:
+                g = gone.get(i, set())
+                maybe[j] = np.array([int(x) in g for x in keys], dtype=bool)
+                must[j] = np.array([int(x) in present[i] for x in keys], dtype=bool) & ~maybe[j]
+        for j in idx:
+            i, k, keys = plan[j]
+            if k == 2:
+                present[i].difference_update(map(int, keys))
+                count[i] -= keys.size
+        for j in idx:
+            if plan[j][1] != 0:
+                occ[j] = count[plan[j][0]]
+    return occ, must, maybe, count
+
+
+def test_server_phased_order_in_one_filter(cf, oracle_mod):
+    """The phased ordering on a short stream: whatever windows the server forms,
+    each window runs a filter's inserts, then its lookups and deletes."""
+    import torch
+
+    f = cf.CuckooFilter(1 << 16, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    g = cf.CuckooFilter(1 << 16, key_bytes=8, bits_per_tag=16, eviction="bfs")
+    kh = oracle_mod.gen_keys(78, 30000)
+    k = cf.keys_to_device(kh)
+    outs = [torch.zeros(k.numel(), dtype=torch.bool, device="cuda") for _ in range(4)]
+    reqs = [(f, cf.OP_LOOKUP, k, outs[0], False), (f, cf.OP_INSERT, k, None, False),
+            (g, cf.OP_INSERT, k, None, False), (f, cf.OP_LOOKUP, k, outs[1], False),
+            (f, cf.OP_LOOKUP, k, outs[2], False), (f, cf.OP_DELETE, k, None, False),
+            (f, cf.OP_LOOKUP, k, outs[3], False)]
+    plan = [(0 if r[0] is f else 1, r[1], kh) for r in reqs]
+    with cf.Server(ordering="phased") as srv:
+        res = srv.replay(srv.make_requests(reqs), np.zeros(len(reqs)))
+    assert (res["status"] == 0).all()
+    assert int(res["window_levels"].max()) <= 2
+    occ, must, maybe, count = _phase_model(plan, res["window"], [0, 0], [set(), set()])
+    assert list(res["occupied"]) == occ
+    look = [j for j, r in enumerate(reqs) if r[1] == cf.OP_LOOKUP]
+    for o, j in zip(outs, look):
+        got = o.cpu().numpy()
+        assert got[must[j]].all()
+        assert got[~must[j] & ~maybe[j]].sum() <= 60  # absent keys: false positives only
+    assert f.occupiedSlots() == count[0] == 0 and g.occupiedSlots() == count[1] == 30000
+
+
+@pytest.mark.parametrize("max_levels,late", [(None, False), (1, False), (None, True)])
+def test_server_phased_stream_matches_model(cf, oracle_mod, max_levels, late):
+    """A mixed stream of 300 requests on 16 filters under the phased ordering,
+    Poisson arrivals: occupancies equal the phase model's for the windows the
+    server formed, every key the model holds present is found, false positives
+    stay within the closed-form bound, deletes of present keys succeed, and the
+    final tables hold exactly the model's counts."""
+    import torch
+
+    O = oracle_mod
+    rng = np.random.default_rng(21)
+    caps = [int(2 ** x) for x in rng.uniform(13, 19, size=16)]
+    served, _, keysets = _oracle_filters(cf, O, caps, seed=900, load=0.5)
+    present0 = [set(map(int, ks)) for ks in keysets]
+    count0 = [f.occupiedSlots() for f in served]
+    present = [set(p) for p in present0]
+    count = list(count0)
+    plan = []
+    for j in range(300):
+        i = int(rng.integers(len(caps)))
+        kind = int(rng.choice([0, 1, 2], p=[0.3, 0.55, 0.15]))
+        n = int(2 ** rng.uniform(0, 11))
+        if kind == 0:
+            n = max(0, min(n, int(0.9 * served[i].capacity()) - count[i]))
+            keys = O.gen_keys(320_000 + j, n)
+            present[i].update(map(int, keys))
+            count[i] += n
+        elif kind == 1:
+            have = np.fromiter(present[i], dtype=np.uint64)
+            pos = have[rng.integers(0, have.size, n // 2)] if have.size else np.zeros(0, np.uint64)
+            keys = np.concatenate([pos, O.gen_keys(350_000 + j, n - pos.size)])
+        else:
+            have = np.fromiter(present[i], dtype=np.uint64)
+            keys = rng.choice(have, size=min(n, have.size), replace=False) if have.size else np.zeros(0, np.uint64)
+            for x in keys:
+                present[i].discard(int(x))
+            count[i] -= keys.size
+        plan.append((i, kind, keys))
+    reqs = []
+    for j, (i, kind, keys) in enumerate(plan):
+        host = j % 2 == 0
+        if host:
+            k = keys.copy()
+            out = np.zeros(keys.size, dtype=bool) if kind else None
+        else:
+            k = cf.keys_to_device(keys) if keys.size else torch.zeros(0, dtype=torch.int64, device="cuda")
+            out = torch.zeros(max(keys.size, 1), dtype=torch.bool, device="cuda") if kind else None
+        reqs.append((served[i], kind, k, out, host))
+    arrivals = np.cumsum(rng.exponential(1e-4, size=len(reqs)))
+    with cf.Server(ordering="phased", max_window_levels=max_levels, form_late=late) as srv:
+        res = srv.replay(srv.make_requests(reqs), arrivals)
+        st = srv.stats()
+    assert (res["status"] == 0).all()
+    assert int(res["window_levels"].max()) <= (max_levels or 2)
+    assert st["windows"] < len(reqs)  # requests shared windows
+    occ, must, maybe, final = _phase_model(plan, res["window"], count0, present0)
+    np.testing.assert_array_equal(res["occupied"], np.array(occ, dtype=np.uint64))
+    fp, neg = 0, 0
+    for j, ((i, kind, keys), (_, _, _, out, _)) in enumerate(zip(plan, reqs)):
+        if not keys.size or kind == 0:
+            continue
+        got = _out_np(out)[: keys.size]
+        if kind == 2:
+            assert got.all()
+        else:
+            assert got[must[j]].all()
+            absent = ~must[j] & ~maybe[j]
+            fp += int(got[absent].sum())
+            neg += int(absent.sum())
+    # 2 * bucketSize * alpha / 2^bitsPerTag at alpha <= 0.9 (src/main.cu:114-115)
+    p = 2 * 16 * 0.9 / 2 ** 16
+    assert fp <= neg * p + 4 * np.sqrt(neg * p) + 3
+    for f, n in zip(served, final):
+        assert f.occupiedSlots() == n
+        assert_table_count(f, n, f.capacity())
 
 
 def test_server_ticket_api_and_errors(cf, oracle_mod):
