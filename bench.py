@@ -527,6 +527,18 @@ def big_table_section(cf, torch, workloads, dev, log2_slots=30, reps=3):
         res[f"{k}_ms"] = round(ms, 3)
         res[f"{k}_mkeys_s"] = round(units[k] / ms / 1e3, 1)
         res[f"{k}_algorithmic_gbs"] = round(BYTES_PER_KEY[k] * units[k] / (ms / 1e3) / 1e9, 1)
+    # one more fill, untimed, with the eviction-chain histogram on: swaps made
+    # -> inserts placed with that many (bin 0 = placed without eviction)
+    f.clear()
+    f.setEvictionHistogram(True)
+    placed = f.insertMany(keys[:n])
+    hist = f.evictionHistogram()
+    f.setEvictionHistogram(False)
+    evicted = int(hist[1:-1].sum())
+    res["eviction_chains"] = {
+        "placed": placed, "failed": int(hist[-1]),
+        "histogram": {"0": placed - evicted, **{str(c): int(hist[c]) for c in range(1, hist.size - 1) if hist[c]}},
+    }
     del keys, query, out, dout
     f.close()
     torch.cuda.empty_cache()
