@@ -90,8 +90,10 @@ def _primary_share(f, keys_dev, sample):
     return float(hit.mean())
 
 
-def measure(kw, slots, load, seed=1, device="cuda", primary_sample=1 << 16):
-    """One candidate on one workload shape; returns a result dict."""
+def measure(kw, slots, load, seed=1, device="cuda", primary_sample=1 << 16, repeats=1):
+    """One candidate on one workload shape; returns a result dict.  With
+    repeats > 1 each timed call runs that many times (every insert and delete
+    from the same starting state) and the fastest counts."""
     import torch
 
     f = cf.CuckooFilter(slots, config=cf.make_config(**kw))
@@ -109,12 +111,20 @@ def measure(kw, slots, load, seed=1, device="cuda", primary_sample=1 << 16):
         f.insertMany(pos)
         hist = f.evictionHistogram()
         f.setEvictionHistogram(False)
-        f.clear()
-        occ, t_ins = _timed(torch, lambda: f.insertMany(pos))
-        _, t_look = _timed(torch, lambda: f.containsMany(query, out=out))
+        t_ins = t_look = t_del = float("inf")
+        for _ in range(repeats):
+            f.clear()
+            occ, t = _timed(torch, lambda: f.insertMany(pos))
+            t_ins = min(t_ins, t)
+        for _ in range(repeats):
+            t_look = min(t_look, _timed(torch, lambda: f.containsMany(query, out=out))[1])
         fps = int(out[n // 2:].sum().item())
         share = _primary_share(f, pos, min(primary_sample, n))
-        _, t_del = _timed(torch, lambda: f.deleteMany(dels))
+        for r in range(repeats):
+            if r:
+                f.clear()
+                f.insertMany(pos)
+            t_del = min(t_del, _timed(torch, lambda: f.deleteMany(dels))[1])
         bucket_bytes = kw["bucket_size"] * TAG_BYTES[kw["bits_per_tag"]]
         # positives: one bucket, a second when the tag is not in the primary; negatives: two
         lookup_bytes = key_bytes + 1 + bucket_bytes * (0.5 * (1 + (1 - share)) + 0.5 * 2)
@@ -344,12 +354,12 @@ def _marker():
         raise RuntimeError(f"marker kernel failed ({rc})")
 
 
-def sweep(slots, load, key_bytes=8, policy="xor", markers=False, **axes):
+def sweep(slots, load, key_bytes=8, policy="xor", markers=False, repeats=1, **axes):
     """Measures every candidate; yields one result dict per point."""
     for kw in candidates(key_bytes=key_bytes, policy=policy, **axes):
         if markers:
             _marker()
-        yield measure(kw, slots, load)
+        yield measure(kw, slots, load, repeats=repeats)
 
 
 def main(argv=None):
@@ -360,13 +370,14 @@ def main(argv=None):
     ap.add_argument("--policy", default="xor")
     ap.add_argument("--blocks", default="64,128,256,512")
     ap.add_argument("--fpr-budget", type=float, default=None, help="also print the chosen point")
+    ap.add_argument("--repeats", type=int, default=1, help="timed runs per call; the fastest counts")
     ap.add_argument("--markers", action="store_true",
                     help="launch a 4 KiB copy kernel before each point (a boundary in profiler dispatch lists)")
     args = ap.parse_args(argv)
     blocks = tuple(int(b) for b in args.blocks.split(","))
     results = []
     for r in sweep(args.slots, args.load, key_bytes=args.key_bytes, policy=args.policy, markers=args.markers,
-                   blocks=blocks):
+                   repeats=args.repeats, blocks=blocks):
         results.append(r)
         print(json.dumps(r), flush=True)
     if args.fpr_budget is not None:
