@@ -278,12 +278,64 @@ struct FilterOps {
         return detail::casRelaxed(bw + start, cur, replaceTag(cur, lane, tag)) == cur;
     }
 
+    // Buckets of an even number of 32-bit words are scanned in LDS two words
+    // at a time.  Two adjacent little-endian 32-bit words hold their slots
+    // exactly where one 64-bit word would, so one 64-bit access covers two of
+    // the reference's words.  The lane masks are still taken per 32-bit word
+    // (the reference's SWAR, whose lowest flag per word is exact), and the
+    // reference's word order from the start word is kept: when the start
+    // word is the upper half of its pair, the first visit allows that half
+    // only and the scan ends with the pair's lower half.  A 64-bit CAS that
+    // fails because the other half changed rereads and continues, as a lost
+    // CAS does.
+    static constexpr bool kPairScan = sizeof(Word) == 4 && WC % 2 == 0;
+
+    /// Per-lane flags of a pair of 32-bit words: empty lanes, or (Match)
+    /// lanes equal to the tag whose broadcast is `rep`.
+    template <bool Match>
+    static __device__ __forceinline__ uint64_t pairFlags(uint64_t pair, Word rep) {
+        const Word lo = static_cast<Word>(pair);
+        const Word hi = static_cast<Word>(pair >> 32);
+        const Word fl = getZeroMask<Tag, Word>(Match ? (lo ^ rep) : lo);
+        const Word fh = getZeroMask<Tag, Word>(Match ? (hi ^ rep) : hi);
+        return static_cast<uint64_t>(fl) | (static_cast<uint64_t>(fh) << 32);
+    }
+
+    /// tryInsertWordsLds / tryRemoveWordsLds (Remove) over word pairs.
+    template <bool Remove>
+    static __device__ __forceinline__ bool scanPairsLds(Word* bw32, Tag tag) {
+        constexpr uint32_t PC = static_cast<uint32_t>(WC / 2);
+        auto* bw = reinterpret_cast<uint64_t*>(bw32);  // a bucket of an even word count is 8-byte aligned
+        const Word rep = replicateTag<Tag, Word>(tag);
+        const uint32_t start = startWord(tag);
+        const uint32_t first = start / 2;
+        const uint64_t firstAllowed = (start & 1u) ? (~uint64_t(0) << 32) : ~uint64_t(0);
+        const uint32_t visits = PC + (start & 1u);
+        for (uint32_t i = 0; i < visits; ++i) {
+            const uint32_t w = (first + i) & (PC - 1);
+            const uint64_t allowed = i == 0 ? firstAllowed : (i == PC ? ~firstAllowed : ~uint64_t(0));
+            uint64_t cur = bw[w];
+            for (;;) {
+                const uint64_t fl = pairFlags<Remove>(cur, rep) & allowed;
+                if (fl == 0) break;
+                const uint32_t shift = (lowestBit(fl) / static_cast<uint32_t>(BITS)) * static_cast<uint32_t>(BITS);
+                const uint64_t put = Remove ? uint64_t(EMPTY) : static_cast<uint64_t>(tag);
+                const uint64_t desired = (cur & ~(static_cast<uint64_t>(L::fpMask) << shift)) | (put << shift);
+                const uint64_t old = detail::casRelaxed(bw + w, cur, desired);
+                if (old == cur) return true;
+                cur = old;
+            }
+        }
+        return false;
+    }
+
     /// tryInsertAtBucket on a bucket in LDS as the reference scans it
     /// (CuckooFilter.cuh:859-904): word by word from the tag's start word,
     /// reading a word only when the ones before it are full.  In LDS the
     /// reads cost bank cycles, not round trips, so the early exit beats one
     /// whole-bucket read.
     static __device__ __forceinline__ bool tryInsertWordsLds(Word* bw, Tag tag) {
+        if constexpr (kPairScan) return scanPairsLds<false>(bw, tag);
         const uint32_t start = startWord(tag);
 #pragma unroll
         for (uint32_t i = 0; i < WC; ++i) {
@@ -304,6 +356,7 @@ struct FilterOps {
     /// tryRemoveAtBucket on a bucket in LDS, word by word from the start word
     /// (CuckooFilter.cuh:805-848); a tag is most often found in its start word.
     static __device__ __forceinline__ bool tryRemoveWordsLds(Word* bw, Tag tag) {
+        if constexpr (kPairScan) return scanPairsLds<true>(bw, tag);
         const uint32_t start = startWord(tag);
         const Word rep = replicateTag<Tag, Word>(tag);
 #pragma unroll

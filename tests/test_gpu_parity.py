@@ -147,9 +147,11 @@ SEQ_CFGS = [
 @pytest.mark.parametrize("path", ["direct", "binned"])
 @pytest.mark.parametrize("kw", SEQ_CFGS, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
 def test_single_key_calls_bit_exact(cf, oracle_mod, kw, path, monkeypatch):
-    """insertMany's two device paths (one thread per key with device CAS; the
-    bucket-binned LDS path) both reproduce the sequential oracle call by call."""
+    """insertMany's and deleteMany's two device paths (one thread per key with
+    device CAS; the bucket-binned LDS path) both reproduce the sequential
+    oracle call by call."""
     monkeypatch.setenv("CUCKOO_INSERT_PATH", path)
+    monkeypatch.setenv("CUCKOO_DELETE_PATH", path)
     O = oracle_mod
     c = cf.make_config(**kw)
     oc = _oracle_cfg(O, c)
