@@ -374,17 +374,17 @@ class BatchExecutor {
         if (phased) {
             // level 1 for a lookup or delete on a filter that has inserts in
             // the window, else level 0
-            kinds_.clear();
+            hasInserts_.clear();
             filterOf_.resize(count);
             for (size_t i = 0; i < count; ++i) {
-                auto [it, fresh] = seen_.try_emplace(ops[i].occupied, static_cast<uint32_t>(kinds_.size()));
-                if (fresh) kinds_.push_back(0);
+                auto [it, fresh] = seen_.try_emplace(ops[i].occupied, static_cast<uint32_t>(hasInserts_.size()));
+                if (fresh) hasInserts_.push_back(0);
                 filterOf_[i] = it->second;
-                if (ops[i].kind == static_cast<uint32_t>(BatchOpKind::Insert)) kinds_[it->second] = 1;
+                if (ops[i].kind == static_cast<uint32_t>(BatchOpKind::Insert)) hasInserts_[it->second] = 1;
             }
             for (size_t i = 0; i < count; ++i) {
                 const bool insert = ops[i].kind == static_cast<uint32_t>(BatchOpKind::Insert);
-                level_[i] = !insert && kinds_[filterOf_[i]] != 0 ? 1u : 0u;
+                level_[i] = !insert && hasInserts_[filterOf_[i]] != 0 ? 1u : 0u;
                 numLevels = std::max(numLevels, level_[i] + 1);
             }
         }
@@ -570,7 +570,7 @@ class BatchExecutor {
     std::unordered_map<const void*, uint32_t> seen_;
     std::vector<FilterLevelsPod> state_;
     std::vector<uint32_t> level_, opStart_, levelOps_, chunkCum_, cursor_, levelMode_;
-    std::vector<uint32_t> kinds_, filterOf_;  // phased planning: per filter, has inserts; each op's filter
+    std::vector<uint32_t> hasInserts_, filterOf_;  // phased planning: per filter, whether it has inserts; each op's filter
     WindowHeader hdr_{};  // the last planned window's header (its offsets into the device window)
     // graphs by level count
     hipStream_t captureStream_ = nullptr;
