@@ -60,14 +60,15 @@ def _out_np(out):
     return out if isinstance(out, np.ndarray) else out.cpu().numpy()
 
 
-def test_server_lookups_alone_and_in_mix_bit_exact(cf, oracle_mod):
+@pytest.mark.parametrize("ordering", ["arrival", "phased"])
+def test_server_lookups_alone_and_in_mix_bit_exact(cf, oracle_mod, ordering):
     O = oracle_mod
     rng = np.random.default_rng(3)
     caps = [int(2 ** x) for x in rng.uniform(12, 18, size=10)]
     fs, ofs, keysets = _oracle_filters(cf, O, caps, seed=300, load=0.85)
     reqs, idx, queries = _lookup_stream(cf, O, fs, keysets, rng, 300)
     expect = [ofs[i].contains_many(q) for i, q in zip(idx, queries)]
-    with cf.Server() as srv:
+    with cf.Server(ordering=ordering, form_late=ordering == "phased") as srv:
         # alone: one request in flight at a time
         for (f, op, keys, out, host), e in zip(reqs[:60], expect[:60]):
             t = srv.submit(f, op, keys, out, host)
