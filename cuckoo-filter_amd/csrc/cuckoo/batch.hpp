@@ -70,7 +70,6 @@ inline constexpr uint32_t kBatchChunk = 512;        // default keys per workgrou
 inline constexpr uint32_t kBatchChunkLatency = 256;  // one key per thread (serving windows)
 inline constexpr int kBatchBlock = 256;
 inline constexpr uint32_t kMaxGraphLevels = 64;  // deeper windows launch level by level
-inline constexpr uint32_t kSplitLastLevelChunks = 256;  // smallest last level split around beforeLast
 
 /// Start of a window buffer; offsets are bytes from the start of the buffer.
 struct alignas(64) WindowHeader {
@@ -94,10 +93,9 @@ __device__ __forceinline__ T* windowPart(unsigned char* w, uint64_t off) {
 
 /// Who records a level's occupancies.
 enum LevelMode : uint32_t {
-    kLookupsOnly = 0,      // workgroup 0, as the level starts (and the previous level's, if deferred)
-    kMutatesRecords = 1,   // the level mutates; its last workgroup records them
-    kMutatesDeferred = 2,  // the level mutates; the next level (lookups only) records them
-    kMutatesUncounted = 3  // the first launch of a mutating level split in two: the second records
+    kLookupsOnly = 0,     // workgroup 0, as the level starts (and the previous level's, if deferred)
+    kMutatesRecords = 1,  // the level mutates; its last workgroup records them
+    kMutatesDeferred = 2  // the level mutates; the next level (lookups only) records them
 };
 
 /// A level's place in the window, passed by a level-by-level launch so that
@@ -110,8 +108,7 @@ struct LevelArgs {
     const uint32_t* chunkCum;
     uint32_t* done;
     uint32_t p0, p1;    // the level's levelOps positions [p0, p1)
-    uint32_t g0, g1;    // the chunks [g0, g1) this launch runs
-    uint32_t gb;        // the level's first chunk (chunkCum[p0]; g0 unless the level is split)
+    uint32_t g0, g1;    // its chunks [g0, g1)
     uint32_t op0;       // levelOps[p0]
     uint32_t mode;      // LevelMode
     uint32_t r0;        // first levelOps position workgroup 0 records (p0, or the previous level's start)
@@ -171,11 +168,10 @@ __global__ __launch_bounds__(kBatchBlock) void batchLevelKernel(unsigned char* _
             results[op] = __hip_atomic_load(ops[op].occupied, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
-    const uint32_t levelBase = la.valid ? la.gb : g0;
     for (uint32_t g = g0 + blockIdx.x; g < g1; g += gridDim.x) {
         // last position j in [p0, p1) with chunkCum[j] <= g (skips empty batches)
         uint32_t lo = p0;
-        uint32_t base = levelBase;
+        uint32_t base = g0;
         if (!single) {
             uint32_t hi = p1 - 1;
             while (lo < hi) {
@@ -330,33 +326,16 @@ class BatchExecutor {
                 la.op0 = levelOps_[la.p0];
                 la.mode = levelMode_[l];
                 la.r0 = l > 0 && levelMode_[l - 1] == kMutatesDeferred ? opStart_[l - 1] : la.p0;
-                la.gb = la.g0;
                 const uint32_t chunks = la.g1 - la.g0;
-                auto run = [&](const LevelArgs& a) {
-                    const uint32_t c = a.g1 - a.g0;
-                    const uint32_t g = c == 0 ? 1u : (c < grid ? c : grid);
-                    if (chunkKeys == kBatchChunk) {
-                        batchLevelKernel<Config, kBatchChunk><<<g, kBatchBlock, 0, stream>>>(win, l, results, a);
-                    } else {
-                        batchLevelKernel<Config, kBatchChunkLatency><<<g, kBatchBlock, 0, stream>>>(win, l, results, a);
-                    }
-                };
-                if (beforeLast != nullptr && l + 1 == numLevels) {
-                    // A large last level runs as two launches, its last quarter
-                    // after the event, so that the event fires shortly before the
-                    // window ends (the server forms its next window then).  The
-                    // parts are a schedule the level's concurrent batches allow;
-                    // only the second counts workgroups and records.
-                    if (chunks >= kSplitLastLevelChunks) {
-                        LevelArgs first = la;
-                        first.g1 = la.g0 + chunks - chunks / 4;
-                        if (la.mode == kMutatesRecords) first.mode = kMutatesUncounted;
-                        run(first);
-                        la.g0 = first.g1;
-                    }
-                    if ((e = hipEventRecord(beforeLast, stream)) != hipSuccess) return e;
+                const uint32_t g = chunks == 0 ? 1u : (chunks < grid ? chunks : grid);
+                if (beforeLast != nullptr && l + 1 == numLevels &&
+                    (e = hipEventRecord(beforeLast, stream)) != hipSuccess)
+                    return e;
+                if (chunkKeys == kBatchChunk) {
+                    batchLevelKernel<Config, kBatchChunk><<<g, kBatchBlock, 0, stream>>>(win, l, results, la);
+                } else {
+                    batchLevelKernel<Config, kBatchChunkLatency><<<g, kBatchBlock, 0, stream>>>(win, l, results, la);
                 }
-                run(la);
             }
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
