@@ -311,7 +311,7 @@ class BatchExecutor {
             if ((e = hipGraphLaunch(graphs_[numLevels], stream)) != hipSuccess) return e;
         } else {
             // level by level, each with a grid no larger than its chunk count
-            const uint32_t grid = levelGrid(chunkKeys, phased);
+            const uint32_t grid = levelGrid(phased);
             LevelArgs la{};
             la.ops = reinterpret_cast<const BatchOpDesc*>(win + hdr_.opsOff);
             la.levelOps = reinterpret_cast<const uint32_t*>(win + hdr_.levelOpsOff);
@@ -506,18 +506,20 @@ class BatchExecutor {
     /// Level grid: the resident workgroups of the two-keys-per-thread kernel
     /// (graph windows, and the serving loop's arrival-ordered windows: 6 per
     /// CU), and three workgroups per CU for phased windows.  Measured on the
-    /// serving workload (2^30-slot k-mer filter + 64 tenants, phased windows
-    /// of 2^22 keys): 512 / 768 / 1024 / 1536 / 2048 workgroups gave 15.3 /
-    /// 15.7 / 15.1 / 13.3 / 11.9 Gkeys/s (fewer device-scope CAS in flight
-    /// queue less at the memory side).  Arrival-ordered windows hold many
-    /// small levels, whose latency rises when fewer workgroups share a level
-    /// (p99 at half load 0.25-0.39 ms at 1536, 0.75 ms at 768); graph windows
-    /// of the batched workload ran fastest at full residency (23.8 Gkeys/s at
-    /// 1536 against 20.0 at 768).
-    static uint32_t levelGrid(uint32_t chunkKeys, bool phased) {
+    /// serving workload (2^30-slot k-mer filter + 64 tenants), phased windows
+    /// of one key per thread: 512 / 768 / 1024 / 1536 / 2048 workgroups gave
+    /// 15.3 / 15.7 / 15.1 / 13.3 / 11.9 Gkeys/s saturated at 2^22-key windows
+    /// (fewer device-scope CAS in flight queue less at the memory side); of
+    /// two keys per thread, 3 / 4 / 6 per CU gave p99 0.73 / 0.83 / 1.07 ms
+    /// at 16.6 Gkeys/s offered and 23.4 / 24.1 / 23.4 saturated.
+    /// Arrival-ordered windows hold many small levels, whose latency rises
+    /// when fewer workgroups share a level (p99 at half load 0.25-0.39 ms at
+    /// 1536, 0.75 ms at 768); graph windows of the batched workload ran
+    /// fastest at full residency (23.8 Gkeys/s at 1536 against 20.0 at 768).
+    static uint32_t levelGrid(bool phased) {
         constexpr size_t kMany = static_cast<size_t>(1) << 30;
         const uint32_t resident = detail::gridFor(batchLevelKernel<Config, kBatchChunk>, kMany, kBatchBlock);
-        if (chunkKeys == kBatchChunk || !phased) return resident;
+        if (!phased) return resident;
         int dev = 0;
         (void)hipGetDevice(&dev);
         return std::min(resident, 3u * detail::computeUnits(dev));
@@ -537,7 +539,7 @@ class BatchExecutor {
             return e;
         auto* win = static_cast<unsigned char*>(dWin_);
         auto* results = static_cast<unsigned long long*>(dResults_);
-        const uint32_t grid = levelGrid(kBatchChunk, false);
+        const uint32_t grid = levelGrid(false);
         if ((e = hipStreamBeginCapture(captureStream_, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
         for (uint32_t l = 0; l < numLevels; ++l) {
             batchLevelKernel<Config, kBatchChunk><<<grid, kBatchBlock, 0, captureStream_>>>(win, l, results,
