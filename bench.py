@@ -19,7 +19,7 @@ HBM bytes per launch read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
 summary in profiles/hbm_traffic_2p28.md.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-load-sweep]
-                    [--cpu-cuckoo-log2 20,24]
+                    [--no-capacity-sweep] [--cpu-cuckoo-log2 20,24]
 """
 from __future__ import annotations
 
@@ -545,6 +545,73 @@ def big_table_section(cf, torch, workloads, dev, log2_slots=30, reps=3):
     return res
 
 
+def capacity_sweep_section(cf, torch, keys, dev, log2_caps=range(16, 29), iters=10):
+    """The reference's benchmark matrix (benchmark/benchmark_common.cuh:247-384,
+    benchmark/cuckoo_sorted_vs_unsorted.cu:18-106): CuckooConfig<uint64_t, 16,
+    500, 256, 16, XorAltBucketPolicy> with the config's default eviction (DFS)
+    at capacities 2^16..2^28, n = floor(capacity x 0.95) keys.  Per capacity,
+    each figure is the mean of `iters` timed calls after one untimed, HIP
+    events around the synchronous call:
+      insert            clear (untimed), insertMany(n keys)
+      insert_sorted     clear (untimed), insertManySorted(n keys)
+      insert_presorted  clear (untimed), insertKernelSorted over packed tags
+                        computed and sorted once beforehand
+      query             containsMany of the same n keys on the filled table
+      delete            clear + insertMany (untimed), deleteMany of the n keys
+                        with per-key results
+    """
+    n_max = int((1 << max(log2_caps)) * LOAD)
+    res = torch.empty(n_max, dtype=torch.bool, device=dev)
+    rows = {}
+    for lg in log2_caps:
+        cap = 1 << lg
+        n = int(cap * LOAD)
+        k = keys[:n]
+        f = cf.CuckooFilter(cap, key_bytes=8, bits_per_tag=16, bucket_size=16, word_bytes=8, policy="xor",
+                            eviction="dfs", max_evictions=500, block_size=256)
+        packed = f.sortPackedTags(f.computePackedTags(k))
+        out = res[:n]
+
+        def timed(call, prep):
+            ts = []
+            for i in range(iters + 1):
+                prep()
+                torch.cuda.synchronize()
+                ev = _events(torch)
+                ev[0].record()
+                call()
+                ev[1].record()
+                torch.cuda.synchronize()
+                if i:
+                    ts.append(ev[0].elapsed_time(ev[1]))
+            return sum(ts) / len(ts)
+
+        def fill():
+            f.clear()
+            f.insertMany(k)
+
+        ms = {
+            "insert": timed(lambda: f.insertMany(k), f.clear),
+            "insert_sorted": timed(lambda: f.insertManySorted(k), f.clear),
+            "insert_presorted": timed(lambda: f.insertPackedSorted(packed), f.clear),
+        }
+        fill()
+        ms["query"] = timed(lambda: f.containsMany(k, out=out), lambda: None)
+        found = int(out.sum().item())
+        ms["delete"] = timed(lambda: f.deleteMany(k, out=out), fill)
+        row = {"keys": n, "table_bytes": f.sizeInBytes(), "query_found": found}
+        for op, t in ms.items():
+            row[f"{op}_ms"] = round(t, 4)
+            row[f"{op}_mkeys_s"] = round(n / t / 1e3, 1)
+        rows[str(lg)] = row
+        del packed
+        f.close()
+    del res
+    torch.cuda.empty_cache()
+    return {"config": "CuckooConfig<uint64_t, 16, 500, 256, 16, XorAltBucketPolicy> (DFS), load 0.95",
+            "iterations": iters, "by_log2_capacity": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -559,6 +626,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-filter comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency runs")
     ap.add_argument("--no-big-table", action="store_true", help="skip the 2^30-slot (2 GiB) table")
+    ap.add_argument("--no-capacity-sweep", action="store_true",
+                    help="skip the reference's benchmark matrix over capacities 2^16..2^28")
     ap.add_argument("--big-table-only", action="store_true",
                     help="run only the 2^30-slot table's insert / lookup / delete (for profiling it alone)")
     args = ap.parse_args()
@@ -708,6 +777,11 @@ def main():
             load_sweep[str(alpha)] = row
         per_op["insert_sorted_mkeys_s"] = load_sweep["0.95"]["insert_sorted_mkeys_s"]
 
+    capacity_sweep = None
+    if not args.no_capacity_sweep:
+        log("capacity sweep: 2^16..2^28 slots, insert / sorted / presorted / query / delete")
+        capacity_sweep = capacity_sweep_section(cf, torch, keys, dev)
+
     big_table = None
     if not args.no_big_table:
         big_table = big_table_section(cf, torch, workloads, dev)
@@ -745,6 +819,8 @@ def main():
     }
     if load_sweep is not None:
         result["load_sweep"] = load_sweep
+    if capacity_sweep is not None:
+        result["capacity_sweep"] = capacity_sweep
     if big_table is not None:
         result["table_2gib"] = big_table
     if batched is not None:

@@ -163,7 +163,116 @@ __global__ __launch_bounds__(kMaxBlock) void computePackedTagsKernel(
     }
 }
 
+namespace detail {
+
+/// Orders the LDS accesses of one wave's lanes around it (a compiler barrier
+/// with wavefront-scope release and acquire): LDS executes a wave's
+/// operations in program order, so a lane reads what another lane of its
+/// wave wrote before the sync.
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/// LDS through which the lanes of a wave hand each other their bucket,
+/// tag and outcome: one slot per thread of the workgroup.
+struct RunLds {
+    unsigned long long bucket[kMaxBlock];
+    uint32_t tag[kMaxBlock];
+    uint8_t out[kMaxBlock];
+};
+
+/// Primary-bucket attempts of one wave's inserts, shared by the lanes that
+/// insert into the same bucket.  Sorted packed tags put runs of consecutive
+/// lanes on one bucket (about 15 at 95% load with 16-slot buckets); one device
+/// CAS per lane on the same words serialises at the memory side.  Here the
+/// first lane of each run (consecutive active lanes with equal i1) reads the
+/// bucket once, places the run's tags one after another in lane order as
+/// tryInsertAtBucket would (CuckooFilter.cuh:859-904: first word from the
+/// tag's start word with an empty slot, its lowest empty slot), and writes
+/// each changed word with one CAS against the value it read.  That is the
+/// run's inserts done back to back, an order the reference's concurrent
+/// kernel allows.  Lanes exchange data through LDS, not cross-lane
+/// permutes.  Returns for the calling lane: kRunPlaced if its tag was
+/// stored, kRunFull if the bucket had no room for it (the rest of its insert
+/// starts at the secondary bucket), kRunRetry if a word it depended on
+/// changed under the run (its insert starts over from the primary bucket).
+enum RunOutcome : uint32_t { kRunPlaced = 0, kRunFull = 1, kRunRetry = 2 };
+
+template <typename Config>
+__device__ __forceinline__ uint32_t primaryRunInsert(const DeviceView<Config>& v, bool active, size_t i1,
+                                                     typename Config::TagType fp, RunLds& lds) {
+    using Ops = FilterOps<Config>;
+    using Word = typename Config::WordType;
+    using Tag = typename Config::TagType;
+    constexpr uint32_t WC = Ops::WC;
+    constexpr uint8_t kFullWord = 0xff;  // "no word": the bucket was full
+    const uint32_t lane = __lane_id();
+    const uint32_t tid = threadIdx.x;
+    lds.bucket[tid] = i1;
+    lds.tag[tid] = fp;
+    waveSync();
+    // Active lanes are a prefix of the wave: the caller's index is the wave's
+    // uniform base plus the lane, compared against n.
+    const bool head = active && (lane == 0 || lds.bucket[tid - 1] != static_cast<unsigned long long>(i1));
+    const uint64_t act = __ballot(active);
+    const uint64_t heads = __ballot(head);
+    if (head) {
+        const uint64_t upTo = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
+        const uint64_t later = heads & ~upTo;
+        const uint32_t runEnd = later != 0 ? static_cast<uint32_t>(__builtin_ctzll(later))
+                                           : static_cast<uint32_t>(__popcll(act));
+        const uint32_t len = runEnd - lane;
+        Word* bw = Ops::bucketPtr(v, i1);
+        const typename Ops::SnapT orig = loadBucket<Word, WC>(bw);
+        typename Ops::SnapT s = orig;
+        uint64_t dirty = 0;
+        for (uint32_t k = 0; k < len; ++k) {
+            const Tag t = static_cast<Tag>(lds.tag[tid + k]);
+            uint8_t w = kFullWord;
+            const uint64_t m = Ops::emptyWords(s);
+            if (m != 0) {
+                w = static_cast<uint8_t>(firstFrom<WC>(m, Ops::startWord(t)));
+                const Word cur = pick<Word, WC>(s, w);
+                const uint32_t slot = lowestBit(getZeroMask<Tag, Word>(cur)) / Config::bitsPerTag;
+                s = put<Word, WC>(s, w, Ops::replaceTag(cur, slot, t));
+                dirty |= 1ull << w;
+            }
+            lds.out[tid + k] = w;
+        }
+        uint64_t failed = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < WC; ++j) {
+            if ((dirty >> j) & 1) {
+                const Word o = pick<Word, WC>(orig, j);
+                if (casRelaxed(bw + j, o, pick<Word, WC>(s, j)) != o) failed |= 1ull << j;
+            }
+        }
+        for (uint32_t k = 0; k < len; ++k) {
+            const uint8_t w = lds.out[tid + k];
+            uint8_t r;
+            if (w == kFullWord) {
+                r = failed == 0 ? kRunFull : kRunRetry;
+            } else {
+                r = ((failed >> w) & 1) ? kRunRetry : kRunPlaced;
+            }
+            lds.out[tid + k] = r;
+        }
+    }
+    waveSync();
+    const uint32_t r = lds.out[tid];
+    waveSync();  // every lane has read its outcome before the slots are reused
+    return r;
+}
+
+}  // namespace detail
+
 /// insertKernelSorted (CuckooFilter.cuh:1209-1266); `keys` is unused, as there.
+/// Buckets of up to 64 bytes take their primary attempts per run of lanes on
+/// one bucket (detail::primaryRunInsert); the secondary bucket and eviction
+/// follow per lane, as in the reference.  128-byte buckets keep one insert
+/// per lane: two register copies of such a bucket spill to scratch.
 template <typename Config>
 __global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename Config::KeyType* /*keys*/,
                                                                 const PackedTagOf<Config>* __restrict__ packedTags,
@@ -175,15 +284,42 @@ __global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename C
     uint32_t ok = 0;
     uint32_t ev = 0;
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const P packed = packedTags[i];
-        uint32_t chain;
-        const bool s = Ops::insertFrom(v, static_cast<size_t>(packed >> Config::bitsPerTag),
-                                       static_cast<Tag>(packed & fpMask), chain);
+    auto account = [&](bool s, uint32_t chain) {
         ok += s;
         ev += chain;
         if (v.chainHist != nullptr && (chain != 0 || !s)) {
             atomicAdd(&v.chainHist[s ? chain : v.maxEvictions + 1], 1u);
+        }
+    };
+    if constexpr (Ops::kSnapshot && Ops::L::bucketBytes <= 64) {
+        __shared__ detail::RunLds runLds;
+        // every lane of a wave runs the same iterations (the base is uniform
+        // per workgroup), so that the run's lanes meet in the wave operations
+        for (size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
+            const size_t i = base + threadIdx.x;
+            const bool active = i < n;
+            const P packed = active ? packedTags[i] : P(0);
+            const size_t i1 = static_cast<size_t>(packed >> Config::bitsPerTag);
+            const Tag fp = static_cast<Tag>(packed & fpMask);
+            const uint32_t r = detail::primaryRunInsert<Config>(v, active, i1, fp, runLds);
+            if (active) {
+                uint32_t chain = 0;
+                bool s = true;
+                if (r == detail::kRunFull) {
+                    s = Ops::insertAfterPrimary(v, i1, fp, chain);
+                } else if (r == detail::kRunRetry) {
+                    s = Ops::insertFrom(v, i1, fp, chain);
+                }
+                account(s, chain);
+            }
+        }
+    } else {
+        for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+            const P packed = packedTags[i];
+            uint32_t chain;
+            const bool s = Ops::insertFrom(v, static_cast<size_t>(packed >> Config::bitsPerTag),
+                                           static_cast<Tag>(packed & fpMask), chain);
+            account(s, chain);
         }
     }
     detail::blockCountAdd(ok, v.occupied, false);

@@ -137,3 +137,94 @@ def test_presorted_flow_batch(cf, oracle_mod, kw):
         out = torch.zeros(n // 2, dtype=torch.bool, device=dk.device)
         assert f.deleteMany(dk[: n // 2], out=out) == n - n // 2 and out.all().item()
     f.close()
+
+
+# insertKernelSorted places the tags of consecutive lanes that share a primary
+# bucket together (one bucket read, one CAS per changed word); these cases
+# cover runs split over waves, partial waves (block sizes that are not a
+# multiple of 64), buckets of 16 to 64 bytes and the three policies
+RUN_CFGS = [dict(), dict(bits_per_tag=8, word_bytes=4), dict(bits_per_tag=32, bucket_size=16),
+            dict(bits_per_tag=16, bucket_size=32), dict(bits_per_tag=16, bucket_size=8),
+            dict(policy="offset"), dict(policy="addsub")]
+
+
+@pytest.mark.parametrize("block", [64, 100, 256, 1024])
+@pytest.mark.parametrize("kw", RUN_CFGS, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()) or "default")
+def test_presorted_runs_hold_each_bucket_s_tags(cf, oracle_mod, kw, block):
+    """With no bucket receiving more primary tags than it has slots, no insert
+    leaves its primary bucket, whatever the interleaving: after one batch
+    every bucket holds exactly the fingerprints of the keys whose primary it
+    is (as a multiset: the order inside a bucket depends on which run of a
+    bucket split over two waves commits first)."""
+    O = oracle_mod
+    c = cf.make_config(block_size=block, **kw)
+    if not cf.config_supported(c):
+        pytest.skip("configuration not built")
+    oc = O.make_config(**c.as_dict())
+    cap = 1 << 18
+    f = cf.CuckooFilter(cap, config=c)
+    nb = f.getNumBuckets()
+    bs = c.bucket_size
+    # a quarter of the slots; fewer for small buckets and for AddSub, whose
+    # primary buckets are not uniform (half of them receive no key)
+    n = cap // 10 if bs < 16 else (cap // 16 if kw.get("policy") == "addsub" else cap // 4)
+    keys = O.gen_keys(97 + block, n, c.key_bytes)
+    i1, _, fp = O.candidates(oc, nb, keys)
+    i1 = i1.astype(np.int64)
+    per = np.bincount(i1, minlength=nb)
+    assert per.max() <= bs, "the keys must not overfill a primary bucket"
+    dk = cf.keys_to_device(keys)
+    packed = f.sortPackedTags(f.computePackedTags(dk))
+    assert f.insertPackedSorted(packed) == n
+    got = np.sort(O.table_tags(f.exportTable(), oc), axis=1)
+    # expected rows, sorted: the bucket's empty slots (0) first, then its tags
+    order = np.lexsort((fp, i1))
+    b = i1[order]
+    start = np.concatenate([[0], np.cumsum(per)])[:-1]
+    col = bs - per[b] + (np.arange(n) - start[b])
+    want = np.zeros_like(got)
+    want[b, col] = fp[order].astype(got.dtype)
+    mism = np.nonzero((got != want).any(axis=1))[0]
+    assert mism.size == 0, f"{mism.size} buckets differ, first {mism[:5]}"
+    assert f.countOccupiedSlots() == n
+    f.close()
+
+
+@pytest.mark.parametrize("block", [100, 1024])
+@pytest.mark.parametrize("eviction", ["bfs", "dfs"])
+def test_presorted_duplicate_runs_keep_every_copy(cf, oracle_mod, eviction, block):
+    """Keys repeated up to 20 times give runs longer than a 16-slot bucket,
+    many of them crossing a wave: the copies beyond a full primary bucket go
+    on to the secondary bucket and eviction, one lane each.  Twenty copies of
+    one fingerprint crowd its two buckets, so some inserts fail in the
+    sequential oracle too (3-12 of these 15,500); the batch may fail a few
+    more, the table agrees with the counter, and every failure loses at most
+    one stored copy (the tag it drops), so the copies missing across all
+    keys number no more than the failures."""
+    O = oracle_mod
+    c = cf.make_config(eviction=eviction, block_size=block)
+    oc = O.make_config(**c.as_dict())
+    cap = 1 << 16
+    f = cf.CuckooFilter(cap, config=c)
+    nb = f.getNumBuckets()
+    rng = np.random.default_rng(5 + block)
+    base = O.gen_keys(73, 1500, c.key_bytes)
+    reps = rng.integers(1, 21, size=base.size)
+    keys = np.repeat(base, reps)
+    rng.shuffle(keys)
+    n = keys.size
+    of = O.Filter(oc, cap)
+    occ_cpu, _ = of.insert_many(keys[np.argsort(O.packed_tags(oc, nb, keys), kind="stable")])
+    dk = cf.keys_to_device(keys)
+    packed = f.sortPackedTags(f.computePackedTags(dk))
+    occ = f.insertPackedSorted(packed)
+    assert n - occ <= 3 * (n - occ_cpu) + 10, (n - occ, n - occ_cpu)
+    assert_table_count(f, occ, n)
+    i1, i2, fp = O.candidates(oc, nb, base)
+    tags = O.table_tags(f.exportTable(), oc)
+    fpc = fp.astype(tags.dtype)[:, None]
+    copies = (tags[i1.astype(np.int64)] == fpc).sum(axis=1)
+    copies += np.where(i2 != i1, (tags[i2.astype(np.int64)] == fpc).sum(axis=1), 0)
+    missing = int(np.maximum(reps - copies, 0).sum())
+    assert missing <= n - occ, (missing, n - occ)
+    f.close()
