@@ -183,6 +183,16 @@ struct RunLds {
     uint8_t out[kMaxBlock];
 };
 
+/// Per wave, the inserts whose primary attempt did not finish (64 slots per
+/// wave, at the wave's slots of the workgroup): they continue together, one
+/// per lane, once 64 have gathered, rather than a few lanes at a time while
+/// the rest of the wave waits.
+struct PendingLds {
+    unsigned long long bucket[kMaxBlock];
+    uint32_t tag[kMaxBlock];
+    uint8_t kind[kMaxBlock];
+};
+
 /// Primary-bucket attempts of one wave's inserts, shared by the lanes that
 /// insert into the same bucket.  Sorted packed tags put runs of consecutive
 /// lanes on one bucket (about 15 at 95% load with 16-slot buckets); one device
@@ -293,6 +303,32 @@ __global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename C
     };
     if constexpr (Ops::kSnapshot && Ops::L::bucketBytes <= 64) {
         __shared__ detail::RunLds runLds;
+        __shared__ detail::PendingLds pending;
+        const uint32_t lane = __lane_id();
+        const uint32_t waveSlots = threadIdx.x - lane;  // the wave's first slot
+        // lanes present in this wave (fewer than 64 in the last wave of a
+        // workgroup whose size is not a multiple of 64)
+        const uint32_t waveLanes = static_cast<uint32_t>(__popcll(__ballot(1)));
+        uint32_t queued = 0;  // wave-uniform
+        // the queued inserts, one per lane: the secondary bucket and eviction
+        // for those whose primary bucket was full, the whole insert again for
+        // those whose word changed under their run
+        auto drain = [&](uint32_t count) {
+            detail::waveSync();
+            for (uint32_t first = 0; first < count; first += waveLanes) {
+                const uint32_t q = first + lane;
+                if (q < count) {
+                    const size_t b = static_cast<size_t>(pending.bucket[waveSlots + q]);
+                    const Tag t = static_cast<Tag>(pending.tag[waveSlots + q]);
+                    uint32_t chain = 0;
+                    const bool s = pending.kind[waveSlots + q] == detail::kRunFull
+                                       ? Ops::insertAfterPrimary(v, b, t, chain)
+                                       : Ops::insertFrom(v, b, t, chain);
+                    account(s, chain);
+                }
+            }
+            detail::waveSync();  // the slots are free again
+        };
         // every lane of a wave runs the same iterations (the base is uniform
         // per workgroup), so that the run's lanes meet in the wave operations
         for (size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
@@ -302,17 +338,24 @@ __global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename C
             const size_t i1 = static_cast<size_t>(packed >> Config::bitsPerTag);
             const Tag fp = static_cast<Tag>(packed & fpMask);
             const uint32_t r = detail::primaryRunInsert<Config>(v, active, i1, fp, runLds);
-            if (active) {
-                uint32_t chain = 0;
-                bool s = true;
-                if (r == detail::kRunFull) {
-                    s = Ops::insertAfterPrimary(v, i1, fp, chain);
-                } else if (r == detail::kRunRetry) {
-                    s = Ops::insertFrom(v, i1, fp, chain);
-                }
-                account(s, chain);
+            if (active && r == detail::kRunPlaced) account(true, 0);
+            const bool defer = active && r != detail::kRunPlaced;
+            const uint64_t deferred = __ballot(defer);
+            const uint32_t count = static_cast<uint32_t>(__popcll(deferred));
+            if (queued + count > 64) {
+                drain(queued);
+                queued = 0;
             }
+            if (defer) {
+                const uint32_t slot = waveSlots + queued +
+                                      static_cast<uint32_t>(__popcll(deferred & ((1ull << lane) - 1)));
+                pending.bucket[slot] = i1;
+                pending.tag[slot] = fp;
+                pending.kind[slot] = static_cast<uint8_t>(r);
+            }
+            queued += count;
         }
+        if (queued != 0) drain(queued);
     } else {
         for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
             const P packed = packedTags[i];
