@@ -728,8 +728,9 @@ inline int log2Floor(size_t v) {
 }
 
 /// Insert path selection: CUCKOO_INSERT_PATH = "direct" | "binned" |
-/// "presorted" | "auto" (default auto: binned once the batch has at least one
-/// key per kBinnedMinBucketsPerKey buckets).  "presorted" sends
+/// "presorted" | "auto" (default auto: binned once the batch holds at least
+/// kBinnedMinInsertKeys keys and one key per kBinnedMinBucketsPerKey
+/// buckets).  "presorted" sends
 /// insertManySorted down the reference's own route -- packed tags, a full
 /// ascending sort, insertKernelSorted -- at every table size.
 enum class InsertPath { Auto, Direct, Binned, Presorted };
@@ -744,7 +745,8 @@ inline InsertPath insertPathFromEnv() {
 }
 
 /// Delete path selection: CUCKOO_DELETE_PATH = "direct" | "binned" | "auto"
-/// (default auto: binned from one key per kBinnedMinBucketsPerKey buckets).
+/// (default auto: binned from kBinnedMinDeleteKeys keys and one key per
+/// kBinnedMinBucketsPerKey buckets).
 inline InsertPath deletePathFromEnv() {
     const char* e = std::getenv("CUCKOO_DELETE_PATH");
     if (e == nullptr) return InsertPath::Auto;
@@ -758,6 +760,16 @@ inline InsertPath deletePathFromEnv() {
 /// slots, the two paths cost the same at about one key per two buckets for
 /// fills of 0.3 to 0.8 (the binned path wins from about one per three at 0.9).
 inline constexpr size_t kBinnedMinBucketsPerKey = 2;
+
+/// Smallest batches the binned paths take by default: below them the direct
+/// kernels are faster at any table size, the binned pipeline's dozen launches
+/// outweighing its savings.  Measured on the reference's benchmark matrix
+/// (n = 95% of 2^16..2^28 slots): the binned insert breaks even at about 2^20
+/// keys (5.2 Gkeys/s either way at 2^20 slots; direct 1.9x faster at 2^16),
+/// the binned delete between 1M and 2M keys (direct 16% faster at 996k keys,
+/// 2.4x at 62k; binned 6% faster at 1.99M).
+inline constexpr size_t kBinnedMinInsertKeys = size_t(1) << 20;
+inline constexpr size_t kBinnedMinDeleteKeys = size_t(3) << 19;
 
 /// CUCKOO_PARTITION=lsd keeps the one-level-per-pass LSD partition even where
 /// the two-level one applies (for comparisons).

@@ -46,7 +46,7 @@ struct Engine {
             case detail::InsertPath::Binned:
                 return true;
             default:
-                return n * detail::kBinnedMinBucketsPerKey >= numBuckets;
+                return n >= detail::kBinnedMinInsertKeys && n * detail::kBinnedMinBucketsPerKey >= numBuckets;
         }
     }
 
@@ -86,7 +86,7 @@ struct Engine {
             case detail::InsertPath::Binned:
                 return true;
             default:
-                return n * detail::kBinnedMinBucketsPerKey >= numBuckets;
+                return n >= detail::kBinnedMinDeleteKeys && n * detail::kBinnedMinBucketsPerKey >= numBuckets;
         }
     }
 
@@ -162,15 +162,21 @@ struct Engine {
 
     /// insertManySorted (CuckooFilter.cuh:475-508): keys are radix-partitioned
     /// by primary bucket (as packed into PackedTagType) and inserted slice by
-    /// slice through LDS.  Tables beyond 2^32 buckets, and every table when
-    /// CUCKOO_INSERT_PATH=presorted, take the reference's own route instead:
-    /// packed tags, sorted ascending over all their bits, then
+    /// slice through LDS; batches too small for that (useBinned) go one
+    /// thread per key, as insertMany's do.  Tables beyond 2^32 buckets, and
+    /// every table when CUCKOO_INSERT_PATH=presorted, take the reference's own
+    /// route instead: packed tags, sorted ascending over all their bits, then
     /// insertKernelSorted.
     static hipError_t insertSorted(const View& v, const Key* keys, size_t n, uint32_t block, hipStream_t s,
                                    Workspace& ws) {
         if (n == 0) return hipSuccess;
         const bool presorted = detail::insertPathFromEnv() == detail::InsertPath::Presorted;
-        if (!presorted && Binned::applicable(v.numBuckets)) return Binned::template run<true>(v, keys, n, block, s, ws);
+        if (!presorted && Binned::applicable(v.numBuckets)) {
+            if (useBinned(n, v.numBuckets)) return Binned::template run<true>(v, keys, n, block, s, ws);
+            auto k = insertKernel<Config>;
+            k<<<detail::gridFor(k, n, block), block, 0, s>>>(keys, n, v);
+            return hipGetLastError();
+        }
         constexpr size_t kChunk = size_t(1) << 31;
         for (size_t off = 0; off < n; off += kChunk) {
             const size_t m = std::min(kChunk, n - off);

@@ -187,7 +187,11 @@ def test_single_key_calls_bit_exact(cf, oracle_mod, kw, path, monkeypatch):
     np.testing.assert_array_equal(f.exportTable(), of.table_bytes())
 
 
-def test_single_key_sorted_calls_bit_exact(cf, oracle_mod):
+@pytest.mark.parametrize("path", ["binned", "auto"])
+def test_single_key_sorted_calls_bit_exact(cf, oracle_mod, path, monkeypatch):
+    """insertManySorted one key per call, down the binned path (forced) and
+    the default one (single keys go one thread per key)."""
+    monkeypatch.setenv("CUCKOO_INSERT_PATH", path)
     O = oracle_mod
     c = cf.make_config(bits_per_tag=16, eviction="bfs", max_evictions=40)
     oc = _oracle_cfg(O, c)
@@ -237,8 +241,7 @@ def test_batch_insert_invariants_and_binomial_parity(cf, oracle_mod, kw, mode, m
     path, and insertManySorted down the reference's own presorted route
     (packed tags, full sort, insertKernelSorted)."""
     sorted_insert = mode in ("sorted", "presorted")
-    if mode != "sorted":
-        monkeypatch.setenv("CUCKOO_INSERT_PATH", mode)
+    monkeypatch.setenv("CUCKOO_INSERT_PATH", "binned" if mode == "sorted" else mode)
     O = oracle_mod
     c = cf.make_config(**kw)
     oc = _oracle_cfg(O, c)
