@@ -19,7 +19,7 @@ HBM bytes per launch read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
 summary in profiles/hbm_traffic_2p28.md.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-load-sweep]
-                    [--no-capacity-sweep] [--cpu-cuckoo-log2 20,24]
+                    [--no-capacity-sweep] [--no-eviction-sweep] [--cpu-cuckoo-log2 20,24]
 """
 from __future__ import annotations
 
@@ -612,6 +612,55 @@ def capacity_sweep_section(cf, torch, keys, dev, log2_caps=range(16, 29), iters=
             "iterations": iters, "by_log2_capacity": rows}
 
 
+EVICTION_LOADS = (0.76, 0.78, 0.80, 0.82, 0.84, 0.86, 0.88, 0.90, 0.92, 0.94, 0.95, 0.96, 0.97, 0.98, 0.99,
+                  0.995, 0.999)
+
+
+def eviction_sweep_section(cf, torch, keys, dev, iters=3):
+    """The reference's eviction study (benchmark/eviction_benchmark.cu:14-166):
+    CuckooConfig<uint64_t, 16, 500, 256, 16, XorAltBucketPolicy, BFS | DFS> at
+    2^28 slots; per load factor L of 0.76..0.999, 75% of the L x capacity keys
+    are inserted untimed, the eviction counter is reset, and the other 25% are
+    inserted by one timed insertMany.  Per point: that insert's Mkeys/s, the
+    evictions it made (swaps, as the reference counts them) and per key, and
+    the keys it failed to place; the mean of `iters` runs after one untimed."""
+    rows = {}
+    for ev in ("bfs", "dfs"):
+        f = cf.CuckooFilter(CAPACITY, key_bytes=8, bits_per_tag=16, bucket_size=16, word_bytes=8, policy="xor",
+                            eviction=ev, max_evictions=500, block_size=256)
+        for load in EVICTION_LOADS:
+            total = int(CAPACITY * load)
+            pre = int(total * 0.75)
+            measured = total - pre
+            ts, evs, fails = [], [], []
+            for i in range(iters + 1):
+                f.clear()
+                occ_pre = f.insertMany(keys[:pre])
+                f.resetEvictionCount()
+                torch.cuda.synchronize()
+                e = _events(torch)
+                e[0].record()
+                occ = f.insertMany(keys[pre:total])
+                e[1].record()
+                torch.cuda.synchronize()
+                if i:
+                    ts.append(e[0].elapsed_time(e[1]))
+                    evs.append(f.evictionCount())
+                    fails.append(measured - (occ - occ_pre))
+            ms = sum(ts) / len(ts)
+            mean_ev = sum(evs) / len(evs)
+            rows.setdefault(ev, {})[str(load)] = {
+                "measured_keys": measured, "mkeys_s": round(measured / ms / 1e3, 1), "ms": round(ms, 3),
+                "evictions": int(mean_ev), "evictions_per_insert": round(mean_ev / measured, 4),
+                "failed": int(sum(fails) / len(fails)),
+            }
+        f.close()
+    torch.cuda.empty_cache()
+    return {"config": "CuckooConfig<uint64_t, 16, 500, 256, 16, XorAltBucketPolicy, BFS | DFS>, 2^28 slots, "
+                      "prefill 75% of each load untimed, the other 25% timed",
+            "iterations": iters, "by_policy": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -628,6 +677,8 @@ def main():
     ap.add_argument("--no-big-table", action="store_true", help="skip the 2^30-slot (2 GiB) table")
     ap.add_argument("--no-capacity-sweep", action="store_true",
                     help="skip the reference's benchmark matrix over capacities 2^16..2^28")
+    ap.add_argument("--no-eviction-sweep", action="store_true",
+                    help="skip the BFS / DFS eviction study over load factors 0.76..0.999")
     ap.add_argument("--big-table-only", action="store_true",
                     help="run only the 2^30-slot table's insert / lookup / delete (for profiling it alone)")
     args = ap.parse_args()
@@ -782,6 +833,11 @@ def main():
         log("capacity sweep: 2^16..2^28 slots, insert / sorted / presorted / query / delete")
         capacity_sweep = capacity_sweep_section(cf, torch, keys, dev)
 
+    eviction_sweep = None
+    if not args.no_eviction_sweep:
+        log("eviction sweep: BFS / DFS at loads 0.76..0.999, 2^28 slots")
+        eviction_sweep = eviction_sweep_section(cf, torch, keys, dev)
+
     big_table = None
     if not args.no_big_table:
         big_table = big_table_section(cf, torch, workloads, dev)
@@ -821,6 +877,8 @@ def main():
         result["load_sweep"] = load_sweep
     if capacity_sweep is not None:
         result["capacity_sweep"] = capacity_sweep
+    if eviction_sweep is not None:
+        result["eviction_sweep"] = eviction_sweep
     if big_table is not None:
         result["table_2gib"] = big_table
     if batched is not None:
