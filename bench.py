@@ -19,7 +19,8 @@ HBM bytes per launch read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
 summary in profiles/hbm_traffic_2p28.md.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-load-sweep]
-                    [--no-capacity-sweep] [--no-eviction-sweep] [--cpu-cuckoo-log2 20,24]
+                    [--no-capacity-sweep] [--no-eviction-sweep]
+                    [--no-policy-sweep] [--cpu-cuckoo-log2 20,24]
 """
 from __future__ import annotations
 
@@ -612,6 +613,63 @@ def capacity_sweep_section(cf, torch, keys, dev, log2_caps=range(16, 29), iters=
             "iterations": iters, "by_log2_capacity": rows}
 
 
+def policy_sweep_section(cf, torch, keys, n_pos, dev, iters=3):
+    """The reference's alternate-bucket policy comparison
+    (benchmark/bucket_policy_benchmark.cu): CuckooConfig<uint64_t, 16, 500,
+    128, 16, Xor | AddSub | Offset> at 2^28 slots and 95% load.  Per policy:
+    insertMany of the n keys (clear untimed), containsMany of the same n keys
+    (all present) and of 2^26 keys never inserted, and deleteMany of the n keys
+    (refill untimed); each the mean of `iters` timed calls after one untimed.
+    At 2^28 slots every policy gets 2^24 buckets, so the load is 95% for
+    each."""
+    n = n_pos
+    neg = keys[n:n + (1 << 26)]
+    res = torch.empty(n, dtype=torch.bool, device=dev)
+    rows = {}
+    for pol in ("xor", "addsub", "offset"):
+        f = cf.CuckooFilter(CAPACITY, key_bytes=8, bits_per_tag=16, bucket_size=16, word_bytes=8, policy=pol,
+                            eviction="dfs", max_evictions=500, block_size=128)
+        k = keys[:n]
+
+        def timed(call, prep):
+            ts = []
+            for i in range(iters + 1):
+                prep()
+                torch.cuda.synchronize()
+                ev = _events(torch)
+                ev[0].record()
+                call()
+                ev[1].record()
+                torch.cuda.synchronize()
+                if i:
+                    ts.append(ev[0].elapsed_time(ev[1]))
+            return sum(ts) / len(ts)
+
+        def fill():
+            f.clear()
+            f.insertMany(k)
+
+        ms = {"insert": timed(lambda: f.insertMany(k), f.clear)}
+        fill()
+        placed = f.occupiedSlots()
+        ms["query"] = timed(lambda: f.containsMany(k, out=res), lambda: None)
+        found = int(res.sum().item())
+        ms["query_negative"] = timed(lambda: f.containsMany(neg, out=res[:neg.numel()]), lambda: None)
+        fpr = int(res[:neg.numel()].sum().item()) / neg.numel()
+        ms["delete"] = timed(lambda: f.deleteMany(k, out=res), fill)
+        units = {"insert": n, "query": n, "query_negative": neg.numel(), "delete": n}
+        row = {"buckets": f.getNumBuckets(), "load": round(n / (f.getNumBuckets() * 16), 4), "placed": placed,
+               "query_found": found, "false_positive_rate": round(fpr, 7)}
+        for op, t in ms.items():
+            row[f"{op}_mkeys_s"] = round(units[op] / t / 1e3, 1)
+        rows[pol] = row
+        f.close()
+    del res
+    torch.cuda.empty_cache()
+    return {"config": "CuckooConfig<uint64_t, 16, 500, 128, 16, Xor | AddSub | Offset> (DFS), 2^28 slots",
+            "keys": n, "iterations": iters, "by_policy": rows}
+
+
 EVICTION_LOADS = (0.76, 0.78, 0.80, 0.82, 0.84, 0.86, 0.88, 0.90, 0.92, 0.94, 0.95, 0.96, 0.97, 0.98, 0.99,
                   0.995, 0.999)
 
@@ -679,6 +737,8 @@ def main():
                     help="skip the reference's benchmark matrix over capacities 2^16..2^28")
     ap.add_argument("--no-eviction-sweep", action="store_true",
                     help="skip the BFS / DFS eviction study over load factors 0.76..0.999")
+    ap.add_argument("--no-policy-sweep", action="store_true",
+                    help="skip the XOR / AddSub / Offset alternate-bucket policy comparison")
     ap.add_argument("--big-table-only", action="store_true",
                     help="run only the 2^30-slot table's insert / lookup / delete (for profiling it alone)")
     args = ap.parse_args()
@@ -838,6 +898,11 @@ def main():
         log("eviction sweep: BFS / DFS at loads 0.76..0.999, 2^28 slots")
         eviction_sweep = eviction_sweep_section(cf, torch, keys, dev)
 
+    policy_sweep = None
+    if not args.no_policy_sweep:
+        log("policy sweep: XOR / AddSub / Offset at 2^28 slots")
+        policy_sweep = policy_sweep_section(cf, torch, keys, n_ins, dev)
+
     big_table = None
     if not args.no_big_table:
         big_table = big_table_section(cf, torch, workloads, dev)
@@ -879,6 +944,8 @@ def main():
         result["capacity_sweep"] = capacity_sweep
     if eviction_sweep is not None:
         result["eviction_sweep"] = eviction_sweep
+    if policy_sweep is not None:
+        result["policy_sweep"] = policy_sweep
     if big_table is not None:
         result["table_2gib"] = big_table
     if batched is not None:
