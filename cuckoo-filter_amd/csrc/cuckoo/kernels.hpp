@@ -52,7 +52,101 @@ __device__ __forceinline__ void blockCountAdd(uint32_t local, unsigned long long
 
 }  // namespace detail
 
-/// insertKernel (CuckooFilter.cuh:1119-1141).
+namespace detail {
+
+/// Orders the LDS accesses of one wave's lanes around it (a compiler barrier
+/// with wavefront-scope release and acquire): LDS executes a wave's
+/// operations in program order, so a lane reads what another lane of its
+/// wave wrote before the sync.
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/// How an insert's attempt at its primary bucket ended: stored; the bucket
+/// was full (the insert goes on at the secondary bucket, then eviction); or
+/// the bucket changed under the attempt (the insert starts over).
+enum RunOutcome : uint32_t { kRunPlaced = 0, kRunFull = 1, kRunRetry = 2 };
+
+/// LDS slots of a workgroup's pending inserts: 64 per wave, at the wave's
+/// slots of the workgroup.
+struct PendingLds {
+    unsigned long long bucket[kMaxBlock];
+    uint32_t tag[kMaxBlock];
+    uint8_t kind[kMaxBlock];
+};
+
+/// One wave's inserts whose primary attempt did not finish.  They continue
+/// together, one per lane, once 64 have gathered (and at the end), instead of
+/// a few lanes at a time while the rest of the wave waits for them: an
+/// insert that goes on to eviction holds its wave for a chain of dependent
+/// bucket reads.  Continuing an insert later is an order the reference's
+/// concurrent kernel allows.  push() and finish() are called by every lane of
+/// the wave at the same points.
+template <typename Config>
+struct PendingInserts {
+    using Ops = FilterOps<Config>;
+    using Tag = typename Config::TagType;
+    PendingLds& q;
+    uint32_t lane;
+    uint32_t waveSlots;  // the wave's first slot
+    uint32_t waveLanes;  // lanes present (fewer than 64 in the last wave of an odd-sized workgroup)
+    uint32_t queued = 0;  // wave-uniform
+
+    __device__ explicit PendingInserts(PendingLds& lds)
+        : q(lds),
+          lane(__lane_id()),
+          waveSlots(threadIdx.x - __lane_id()),
+          waveLanes(static_cast<uint32_t>(__popcll(__ballot(1)))) {}
+
+    template <typename Account>
+    __device__ void push(const DeviceView<Config>& v, bool defer, size_t bucket, Tag tag, uint32_t kind,
+                         Account& account) {
+        const uint64_t deferred = __ballot(defer);
+        const uint32_t count = static_cast<uint32_t>(__popcll(deferred));
+        if (queued + count > 64) drain(v, account);
+        if (defer) {
+            const uint32_t slot = waveSlots + queued + static_cast<uint32_t>(__popcll(deferred & ((1ull << lane) - 1)));
+            q.bucket[slot] = bucket;
+            q.tag[slot] = tag;
+            q.kind[slot] = static_cast<uint8_t>(kind);
+        }
+        queued += count;
+    }
+
+    template <typename Account>
+    __device__ void finish(const DeviceView<Config>& v, Account& account) {
+        if (queued != 0) drain(v, account);
+    }
+
+   private:
+    template <typename Account>
+    __device__ void drain(const DeviceView<Config>& v, Account& account) {
+        waveSync();
+        for (uint32_t first = 0; first < queued; first += waveLanes) {
+            const uint32_t k = first + lane;
+            if (k < queued) {
+                const size_t b = static_cast<size_t>(q.bucket[waveSlots + k]);
+                const Tag t = static_cast<Tag>(q.tag[waveSlots + k]);
+                uint32_t chain = 0;
+                const bool s = q.kind[waveSlots + k] == kRunFull ? Ops::insertAfterPrimary(v, b, t, chain)
+                                                                 : Ops::insertFrom(v, b, t, chain);
+                account(s, chain);
+            }
+        }
+        waveSync();  // the slots are free again
+        queued = 0;
+    }
+};
+
+}  // namespace detail
+
+/// insertKernel (CuckooFilter.cuh:1119-1141).  One insert per thread:
+/// unlike the sorted kernel's runs, random keys spread a wave's work over
+/// unrelated buckets, and queueing the inserts that go on to eviction
+/// (detail::PendingInserts) made this kernel slower (14.4 -> 13.8 Gkeys/s at
+/// 2^28 slots, 95%): other waves already hide one wave's eviction chains.
 template <typename Config>
 __global__ __launch_bounds__(kMaxBlock) void insertKernel(const typename Config::KeyType* __restrict__ keys,
                                                           size_t n, DeviceView<Config> v) {
@@ -165,32 +259,12 @@ __global__ __launch_bounds__(kMaxBlock) void computePackedTagsKernel(
 
 namespace detail {
 
-/// Orders the LDS accesses of one wave's lanes around it (a compiler barrier
-/// with wavefront-scope release and acquire): LDS executes a wave's
-/// operations in program order, so a lane reads what another lane of its
-/// wave wrote before the sync.
-__device__ __forceinline__ void waveSync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 /// LDS through which the lanes of a wave hand each other their bucket,
 /// tag and outcome: one slot per thread of the workgroup.
 struct RunLds {
     unsigned long long bucket[kMaxBlock];
     uint32_t tag[kMaxBlock];
     uint8_t out[kMaxBlock];
-};
-
-/// Per wave, the inserts whose primary attempt did not finish (64 slots per
-/// wave, at the wave's slots of the workgroup): they continue together, one
-/// per lane, once 64 have gathered, rather than a few lanes at a time while
-/// the rest of the wave waits.
-struct PendingLds {
-    unsigned long long bucket[kMaxBlock];
-    uint32_t tag[kMaxBlock];
-    uint8_t kind[kMaxBlock];
 };
 
 /// Primary-bucket attempts of one wave's inserts, shared by the lanes that
@@ -208,7 +282,6 @@ struct PendingLds {
 /// stored, kRunFull if the bucket had no room for it (the rest of its insert
 /// starts at the secondary bucket), kRunRetry if a word it depended on
 /// changed under the run (its insert starts over from the primary bucket).
-enum RunOutcome : uint32_t { kRunPlaced = 0, kRunFull = 1, kRunRetry = 2 };
 
 template <typename Config>
 __device__ __forceinline__ uint32_t primaryRunInsert(const DeviceView<Config>& v, bool active, size_t i1,
@@ -303,32 +376,8 @@ __global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename C
     };
     if constexpr (Ops::kSnapshot && Ops::L::bucketBytes <= 64) {
         __shared__ detail::RunLds runLds;
-        __shared__ detail::PendingLds pending;
-        const uint32_t lane = __lane_id();
-        const uint32_t waveSlots = threadIdx.x - lane;  // the wave's first slot
-        // lanes present in this wave (fewer than 64 in the last wave of a
-        // workgroup whose size is not a multiple of 64)
-        const uint32_t waveLanes = static_cast<uint32_t>(__popcll(__ballot(1)));
-        uint32_t queued = 0;  // wave-uniform
-        // the queued inserts, one per lane: the secondary bucket and eviction
-        // for those whose primary bucket was full, the whole insert again for
-        // those whose word changed under their run
-        auto drain = [&](uint32_t count) {
-            detail::waveSync();
-            for (uint32_t first = 0; first < count; first += waveLanes) {
-                const uint32_t q = first + lane;
-                if (q < count) {
-                    const size_t b = static_cast<size_t>(pending.bucket[waveSlots + q]);
-                    const Tag t = static_cast<Tag>(pending.tag[waveSlots + q]);
-                    uint32_t chain = 0;
-                    const bool s = pending.kind[waveSlots + q] == detail::kRunFull
-                                       ? Ops::insertAfterPrimary(v, b, t, chain)
-                                       : Ops::insertFrom(v, b, t, chain);
-                    account(s, chain);
-                }
-            }
-            detail::waveSync();  // the slots are free again
-        };
+        __shared__ detail::PendingLds pendingLds;
+        detail::PendingInserts<Config> pending(pendingLds);
         // every lane of a wave runs the same iterations (the base is uniform
         // per workgroup), so that the run's lanes meet in the wave operations
         for (size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
@@ -339,23 +388,9 @@ __global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename C
             const Tag fp = static_cast<Tag>(packed & fpMask);
             const uint32_t r = detail::primaryRunInsert<Config>(v, active, i1, fp, runLds);
             if (active && r == detail::kRunPlaced) account(true, 0);
-            const bool defer = active && r != detail::kRunPlaced;
-            const uint64_t deferred = __ballot(defer);
-            const uint32_t count = static_cast<uint32_t>(__popcll(deferred));
-            if (queued + count > 64) {
-                drain(queued);
-                queued = 0;
-            }
-            if (defer) {
-                const uint32_t slot = waveSlots + queued +
-                                      static_cast<uint32_t>(__popcll(deferred & ((1ull << lane) - 1)));
-                pending.bucket[slot] = i1;
-                pending.tag[slot] = fp;
-                pending.kind[slot] = static_cast<uint8_t>(r);
-            }
-            queued += count;
+            pending.push(v, active && r != detail::kRunPlaced, i1, fp, r, account);
         }
-        if (queued != 0) drain(queued);
+        pending.finish(v, account);
     } else {
         for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
             const P packed = packedTags[i];
