@@ -149,3 +149,53 @@ def test_sweep_point_at_scale(cf, log2_slots, eviction):
     del keys
     torch.cuda.empty_cache()
     f.close()
+
+
+@pytest.mark.timeout(300)
+def test_table_past_2p32_buckets(cf, oracle_mod):
+    """2^33 buckets of four 16-bit slots (a 64 GiB table, XOR): past 2^32
+    buckets the binned paths do not apply, so insertMany and deleteMany run
+    one thread per key and insertManySorted takes the presorted route (49-bit
+    packed tags, insertKernelSorted).  i1 comes from the hash's low word
+    (bucket_policies.cuh:56-57), so it stays below 2^32, while half of the
+    alternate buckets lie past it.  The keys are chosen so that no two share a
+    bucket; each goes in five times, so four copies fill its primary bucket
+    and one lands in its alternate whatever the interleaving.  The device's
+    (i1, i2, fp) equal the oracle's, the counter counts every copy, every key
+    is found, the five deletes per key all succeed and leave the table empty,
+    and the presorted route does the same."""
+    import torch
+
+    O = oracle_mod
+    c = cf.make_config(policy="xor", bits_per_tag=16, bucket_size=4, word_bytes=8, eviction="bfs")
+    oc = O.make_config(**c.as_dict())
+    cap = 1 << 35
+    nb = O.num_buckets(oc, cap)
+    assert nb == 1 << 33
+    pool = O.gen_keys(3301, 1 << 18, c.key_bytes)
+    i1, i2, fp = O.candidates(oc, nb, pool)
+    buckets, counts = np.unique(np.concatenate([i1, i2]), return_counts=True)
+    once = set(buckets[counts == 1].tolist())
+    keep = np.array([a in once and b in once for a, b in zip(i1.tolist(), i2.tolist())])
+    keys, i1, i2, fp = pool[keep], i1[keep], i2[keep], fp[keep]
+    n = keys.size
+    assert n > 1 << 17 and i1.max() < 1 << 32 and (i2 >= 1 << 32).sum() > n // 4
+    negatives = O.gen_keys(3302, n, c.key_bytes)
+    f = cf.CuckooFilter(cap, config=c)
+    assert f.getNumBuckets() == nb
+    dk = cf.keys_to_device(keys)
+    d1, d2, dfp = f.computeCandidates(dk)
+    np.testing.assert_array_equal(d1.cpu().numpy().astype(np.uint64), i1)
+    np.testing.assert_array_equal(d2.cpu().numpy().astype(np.uint64), i2)
+    np.testing.assert_array_equal(dfp.cpu().numpy().astype(np.uint32), fp)
+    five = dk.repeat(5)
+    out = torch.empty(five.numel(), dtype=torch.bool, device=dk.device)
+    for name, insert in (("insertMany", f.insertMany), ("insertManySorted", f.insertManySorted)):
+        print(f"{name}: {five.numel()} inserts into 2^33 buckets", flush=True)
+        assert insert(five) == 5 * n, name
+        assert f.containsMany(dk).all().item(), name
+        assert int(f.containsMany(cf.keys_to_device(negatives)).sum().item()) <= 5, name
+        assert f.deleteMany(five, out=out) == 0, name
+        assert out.all().item(), name
+        assert not f.containsMany(dk).any().item(), name
+    f.close()
