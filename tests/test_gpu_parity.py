@@ -427,3 +427,38 @@ def test_empty_and_ragged_batches(cf, oracle_mod):
         f.clear()
         assert f.insertMany(_dev_keys(cf, keys)) == n
         assert f.containsMany(_dev_keys(cf, keys)).all().item()
+
+
+@pytest.mark.parametrize("block", [1, 33, 100, 1000, 1024])
+def test_direct_paths_with_any_block_size(cf, oracle_mod, block, monkeypatch):
+    """The config's block size is a run-time value (1..1024) for the
+    one-thread-per-key kernels, and workgroups that are not a multiple of 64
+    leave a partial last wave (blockCountAdd then skips the wave reduction).
+    insertMany and deleteMany down the direct paths and containsMany, at
+    2^16 slots and 90% load: the counter matches the table, every key is
+    found, lookups on the exported image equal the oracle's bit for bit, and
+    every key's delete succeeds, bringing the counter back to 0."""
+    import torch
+
+    monkeypatch.setenv("CUCKOO_INSERT_PATH", "direct")
+    monkeypatch.setenv("CUCKOO_DELETE_PATH", "direct")
+    O = oracle_mod
+    c = cf.make_config(eviction="bfs", block_size=block)
+    oc = _oracle_cfg(O, c)
+    cap = 1 << 16
+    f = cf.CuckooFilter(cap, config=c)
+    n = int(f.capacity() * 0.9)
+    keys = O.gen_keys(700 + block, n + 8192)
+    dk = _dev_keys(cf, keys)
+    occ = f.insertMany(dk[:n])
+    assert occ == n
+    assert_table_count(f, occ, n)
+    got = f.containsMany(dk).cpu().numpy()
+    assert got[:n].all()
+    of = O.Filter(oc, cap)
+    of.load_table(f.exportTable(), occ)
+    np.testing.assert_array_equal(got, of.contains_many(keys))
+    out = torch.empty(n, dtype=torch.bool, device=dk.device)
+    assert f.deleteMany(dk[:n], out=out) == 0
+    assert out.all().item()
+    f.close()
