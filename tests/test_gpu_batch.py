@@ -22,9 +22,17 @@ def _dev(cf, keys):
     return cf.keys_to_device(np.ascontiguousarray(keys))
 
 
-def _make_filters(cf, O, caps, seed, load=0.8):
+# configurations the batched executor is run with: tag widths, word sizes,
+# the three alternate-bucket policies, both eviction policies
+BATCH_CFGS = [dict(eviction="bfs"), dict(eviction="bfs", bits_per_tag=8, word_bytes=4),
+              dict(eviction="dfs", bits_per_tag=32, bucket_size=8), dict(eviction="bfs", policy="addsub"),
+              dict(eviction="dfs", policy="offset")]
+CFG_IDS = ["-".join(f"{k}{v}" for k, v in d.items()) for d in BATCH_CFGS]
+
+
+def _make_filters(cf, O, caps, seed, load=0.8, kw=None):
     """Filters with oracle-built images, and identical twins."""
-    c = cf.make_config(eviction="bfs")
+    c = cf.make_config(**(kw or dict(eviction="bfs")))
     oc = O.make_config(**c.as_dict())
     plain, batched, keysets = [], [], []
     for i, cap in enumerate(caps):
@@ -54,13 +62,14 @@ def _run_plain(ops):
     return np.array(occ, dtype=np.uint64)
 
 
-def test_batch_lookups_bit_exact(cf, oracle_mod):
+@pytest.mark.parametrize("kw", BATCH_CFGS, ids=CFG_IDS)
+def test_batch_lookups_bit_exact(cf, oracle_mod, kw):
     import torch
 
     O = oracle_mod
     rng = np.random.default_rng(1)
     caps = [int(2 ** x) for x in rng.uniform(12, 18, size=12)]
-    plain, batched, keysets = _make_filters(cf, O, caps, seed=100)
+    plain, batched, keysets = _make_filters(cf, O, caps, seed=100, kw=kw)
     ops_p, ops_b, expect = [], [], []
     for j in range(200):
         i = int(rng.integers(len(caps)))
@@ -152,13 +161,14 @@ def test_batch_consecutive_inserts_report_sequential_occupancies(cf, oracle_mod,
     assert_table_count(f, f.occupiedSlots(), f.capacity())
 
 
-def test_batch_mixed_window_matches_plain(cf, oracle_mod):
+@pytest.mark.parametrize("kw", BATCH_CFGS, ids=CFG_IDS)
+def test_batch_mixed_window_matches_plain(cf, oracle_mod, kw):
     import torch
 
     O = oracle_mod
     rng = np.random.default_rng(7)
     caps = [int(2 ** x) for x in rng.uniform(13, 20, size=24)]
-    plain, batched, keysets = _make_filters(cf, O, caps, seed=500, load=0.5)
+    plain, batched, keysets = _make_filters(cf, O, caps, seed=500, load=0.5, kw=kw)
     present = [set(map(int, ks)) for ks in keysets]
     count = [plain[i].occupiedSlots() for i in range(len(caps))]
     ops_p, ops_b, checks = [], [], []
