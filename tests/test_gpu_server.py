@@ -20,11 +20,14 @@ from conftest import assert_table_count
 pytestmark = pytest.mark.gpu
 
 
-def _oracle_filters(cf, O, caps, seed, load):
-    c = cf.make_config(eviction="bfs")
-    oc = O.make_config(**c.as_dict())
+def _oracle_filters(cf, O, caps, seed, load, kws=None):
+    """Filters with oracle-built images; filter i takes configuration
+    kws[i % len(kws)] (default: 16-bit tags, bucket 16, u64 words, BFS)."""
+    kws = kws or [dict(eviction="bfs")]
     fs, ofs, keysets = [], [], []
     for i, cap in enumerate(caps):
+        c = cf.make_config(**kws[i % len(kws)])
+        oc = O.make_config(**c.as_dict())
         of = O.Filter(oc, cap)
         keys = O.gen_keys(seed + i, int(of.capacity() * load))
         occ, _ = of.insert_many(keys)
@@ -92,6 +95,39 @@ def test_server_lookups_alone_and_in_mix_bit_exact(cf, oracle_mod, ordering):
             np.testing.assert_array_equal(_out_np(out), e)
             assert int(r["occupied"]) == f.occupiedSlots()
         assert (res["done_ns"] >= res["start_ns"]).all() and (res["start_ns"] >= res["arrival_ns"]).all()
+
+
+@pytest.mark.parametrize("ordering", ["arrival", "phased"])
+def test_server_filters_of_several_configurations(cf, oracle_mod, ordering):
+    """A window holds requests of one configuration: a stream that
+    alternates between filters of four configurations (tag widths, word
+    sizes, policies) is cut into windows wherever the configuration changes,
+    and every lookup still equals the oracle's, alone and in a burst."""
+    O = oracle_mod
+    rng = np.random.default_rng(11)
+    caps = [int(2 ** x) for x in rng.uniform(12, 17, size=8)]
+    kws = [dict(eviction="bfs"), dict(eviction="bfs", bits_per_tag=8, word_bytes=4),
+           dict(eviction="dfs", bits_per_tag=32, bucket_size=8), dict(eviction="bfs", policy="offset")]
+    fs, ofs, keysets = _oracle_filters(cf, O, caps, seed=700, load=0.85, kws=kws)
+    reqs, idx, queries = _lookup_stream(cf, O, fs, keysets, rng, 200, max_log2=11)
+    expect = [ofs[i].contains_many(q) for i, q in zip(idx, queries)]
+    with cf.Server(ordering=ordering) as srv:
+        for (f, op, keys, out, host), e in zip(reqs[:40], expect[:40]):
+            r = srv.wait(srv.submit(f, op, keys, out, host))
+            assert r["status"] == 0
+            np.testing.assert_array_equal(_out_np(out), e)
+        for _, _, _, out, _ in reqs:
+            if isinstance(out, np.ndarray):
+                out[:] = False
+            else:
+                out.zero_()
+        res = srv.replay(srv.make_requests(reqs), np.zeros(len(reqs)))
+        assert (res["status"] == 0).all()
+        for (f, _, _, out, _), e, r in zip(reqs, expect, res):
+            np.testing.assert_array_equal(_out_np(out), e)
+            assert int(r["occupied"]) == f.occupiedSlots()
+        changes = sum(reqs[j][0].config.as_dict() != reqs[j - 1][0].config.as_dict() for j in range(1, len(reqs)))
+        assert srv.stats()["windows"] >= 40 + changes + 1
 
 
 @pytest.mark.parametrize("max_levels", [None, 1, 2])
