@@ -19,7 +19,7 @@ HBM bytes per launch read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
 summary in profiles/hbm_traffic_2p28.md.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-load-sweep]
-                    [--no-capacity-sweep] [--no-eviction-sweep]
+                    [--no-load-factor] [--no-capacity-sweep] [--no-eviction-sweep]
                     [--no-policy-sweep] [--cpu-cuckoo-log2 20,24]
 """
 from __future__ import annotations
@@ -613,6 +613,72 @@ def capacity_sweep_section(cf, torch, keys, dev, log2_caps=range(16, 29), iters=
             "iterations": iters, "by_log2_capacity": rows}
 
 
+def load_factor_section(cf, torch, keys, dev, loads=range(5, 100, 5), iters=3):
+    """The reference's load-factor benchmark (benchmark/load_factor_benchmark.cu:
+    36, 44, 345-401): CuckooConfig<uint64_t, 16, 500, 128, 16,
+    XorAltBucketPolicy> (DFS) at 2^28 slots, n = floor(2^28 x load) keys for
+    load = 0.05..0.95 in steps of 0.05.  At this n the reference's
+    adaptiveInsert (benchmark/benchmark_common.cuh:114-126) is insertMany.
+    Per load, each figure is the mean of `iters` timed calls after one
+    untimed, HIP events around the synchronous call:
+      insert          clear (untimed), insertMany(n keys)
+      query           containsMany of the same n keys on the filled table
+      query_negative  containsMany of n keys never inserted (the next n keys
+                      of the same bijective stream, so disjoint from the n)
+      delete          clear + insertMany (untimed), deleteMany of the n keys
+    """
+    n_max = int(CAPACITY * max(loads) / 100)
+    assert keys.numel() >= n_max
+    import workloads
+
+    neg_all = workloads.gen_keys_torch(SEED, n_max, start=keys.numel(), device=dev)
+    res = torch.empty(n_max, dtype=torch.bool, device=dev)
+    f = cf.CuckooFilter(CAPACITY, key_bytes=8, bits_per_tag=16, bucket_size=16, word_bytes=8, policy="xor",
+                        eviction="dfs", max_evictions=500, block_size=128)
+    rows = {}
+    for pct in loads:
+        n = int(CAPACITY * pct / 100)
+        k, neg, out = keys[:n], neg_all[:n], res[:n]
+
+        def timed(call, prep):
+            ts = []
+            for i in range(iters + 1):
+                prep()
+                torch.cuda.synchronize()
+                ev = _events(torch)
+                ev[0].record()
+                call()
+                ev[1].record()
+                torch.cuda.synchronize()
+                if i:
+                    ts.append(ev[0].elapsed_time(ev[1]))
+            return sum(ts) / len(ts)
+
+        def fill():
+            f.clear()
+            f.insertMany(k)
+
+        placed = []
+        ms = {"insert": timed(lambda: placed.append(f.insertMany(k)), f.clear)}
+        fill()
+        ms["query"] = timed(lambda: f.containsMany(k, out=out), lambda: None)
+        found = int(out.sum().item())
+        ms["query_negative"] = timed(lambda: f.containsMany(neg, out=out), lambda: None)
+        false_pos = int(out.sum().item())
+        ms["delete"] = timed(lambda: f.deleteMany(k, out=out), fill)
+        row = {"keys": n, "placed": placed[-1], "query_found": found,
+               "false_positive_rate": round(false_pos / n, 7)}
+        for op, t in ms.items():
+            row[f"{op}_ms"] = round(t, 4)
+            row[f"{op}_mkeys_s"] = round(n / t / 1e3, 1)
+        rows[f"{pct / 100:.2f}"] = row
+    f.close()
+    del neg_all, res
+    torch.cuda.empty_cache()
+    return {"config": "CuckooConfig<uint64_t, 16, 500, 128, 16, XorAltBucketPolicy> (DFS), 2^28 slots",
+            "iterations": iters, "by_load": rows}
+
+
 def policy_sweep_section(cf, torch, keys, n_pos, dev, iters=3):
     """The reference's alternate-bucket policy comparison
     (benchmark/bucket_policy_benchmark.cu): CuckooConfig<uint64_t, 16, 500,
@@ -733,6 +799,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-filter comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency runs")
     ap.add_argument("--no-big-table", action="store_true", help="skip the 2^30-slot (2 GiB) table")
+    ap.add_argument("--no-load-factor", action="store_true",
+                    help="skip the reference's load-factor benchmark (loads 0.05..0.95, 2^28 slots)")
     ap.add_argument("--no-capacity-sweep", action="store_true",
                     help="skip the reference's benchmark matrix over capacities 2^16..2^28")
     ap.add_argument("--no-eviction-sweep", action="store_true",
@@ -888,6 +956,11 @@ def main():
             load_sweep[str(alpha)] = row
         per_op["insert_sorted_mkeys_s"] = load_sweep["0.95"]["insert_sorted_mkeys_s"]
 
+    load_factor = None
+    if not args.no_load_factor:
+        log("load factor: loads 0.05..0.95, insert / query / query negative / delete of all n keys")
+        load_factor = load_factor_section(cf, torch, keys, dev)
+
     capacity_sweep = None
     if not args.no_capacity_sweep:
         log("capacity sweep: 2^16..2^28 slots, insert / sorted / presorted / query / delete")
@@ -940,6 +1013,8 @@ def main():
     }
     if load_sweep is not None:
         result["load_sweep"] = load_sweep
+    if load_factor is not None:
+        result["load_factor"] = load_factor
     if capacity_sweep is not None:
         result["capacity_sweep"] = capacity_sweep
     if eviction_sweep is not None:
