@@ -153,10 +153,17 @@ struct Engine {
     /// insertKernelSorted over n packed tags (CuckooFilter.cuh:1209-1266), as
     /// the reference's presorted benchmark launches it
     /// (benchmark/cuckoo_sorted_vs_unsorted.cu:74-76).
+    /// Workgroups of up to kSortedSmallBlock threads take the variant whose
+    /// LDS is sized for them.
     static hipError_t insertPacked(const View& v, const Packed* packed, size_t n, uint32_t block, hipStream_t s) {
         if (n == 0) return hipSuccess;
-        auto k = insertKernelSorted<Config>;
-        k<<<detail::gridFor(k, n, block), block, 0, s>>>(nullptr, packed, n, v);
+        if (block <= kSortedSmallBlock) {
+            auto k = insertKernelSortedSmall<Config>;
+            k<<<detail::gridFor(k, n, block), block, 0, s>>>(packed, n, v);
+        } else {
+            auto k = insertKernelSorted<Config>;
+            k<<<detail::gridFor(k, n, block), block, 0, s>>>(nullptr, packed, n, v);
+        }
         return hipGetLastError();
     }
 

@@ -70,11 +70,12 @@ __device__ __forceinline__ void waveSync() {
 enum RunOutcome : uint32_t { kRunPlaced = 0, kRunFull = 1, kRunRetry = 2 };
 
 /// LDS slots of a workgroup's pending inserts: 64 per wave, at the wave's
-/// slots of the workgroup.
+/// slots of the workgroup (Slots >= blockDim.x).
+template <uint32_t Slots>
 struct PendingLds {
-    unsigned long long bucket[kMaxBlock];
-    uint32_t tag[kMaxBlock];
-    uint8_t kind[kMaxBlock];
+    unsigned long long bucket[Slots];
+    uint32_t tag[Slots];
+    uint8_t kind[Slots];
 };
 
 /// One wave's inserts whose primary attempt did not finish.  They continue
@@ -84,17 +85,17 @@ struct PendingLds {
 /// bucket reads.  Continuing an insert later is an order the reference's
 /// concurrent kernel allows.  push() and finish() are called by every lane of
 /// the wave at the same points.
-template <typename Config>
+template <typename Config, uint32_t Slots>
 struct PendingInserts {
     using Ops = FilterOps<Config>;
     using Tag = typename Config::TagType;
-    PendingLds& q;
+    PendingLds<Slots>& q;
     uint32_t lane;
     uint32_t waveSlots;  // the wave's first slot
     uint32_t waveLanes;  // lanes present (fewer than 64 in the last wave of an odd-sized workgroup)
     uint32_t queued = 0;  // wave-uniform
 
-    __device__ explicit PendingInserts(PendingLds& lds)
+    __device__ explicit PendingInserts(PendingLds<Slots>& lds)
         : q(lds),
           lane(__lane_id()),
           waveSlots(threadIdx.x - __lane_id()),
@@ -259,38 +260,74 @@ __global__ __launch_bounds__(kMaxBlock) void computePackedTagsKernel(
 
 namespace detail {
 
-/// LDS through which the lanes of a wave hand each other their bucket,
-/// tag and outcome: one slot per thread of the workgroup.
+/// LDS through which the lanes of a wave hand each other their bucket, tag,
+/// slot and the run's changed words: one slot per thread of the workgroup
+/// (Slots >= blockDim.x).
+template <uint32_t Slots>
 struct RunLds {
-    unsigned long long bucket[kMaxBlock];
-    uint32_t tag[kMaxBlock];
-    uint8_t out[kMaxBlock];
+    unsigned long long bucket[Slots];   // i1, then the old value of the word this lane writes
+    unsigned long long desired[Slots];  // the new value of the word this lane writes
+    uint32_t tag[Slots];                // the lane's tag, then (at a run's first lane) its changed words
+    uint8_t out[Slots];                 // the bucket slot the lane's tag takes
+    uint8_t word[Slots];                // the word this lane writes
 };
+
+/// Bit s set iff slot s of the bucket snapshot is empty (slot s is lane
+/// s % TPW of word s / TPW).  Exact in every lane, unlike getZeroMask.
+template <typename Config>
+__device__ __forceinline__ uint64_t emptySlots(typename FilterOps<Config>::SnapT s) {
+    using Ops = FilterOps<Config>;
+    using Word = typename Config::WordType;
+    using Tag = typename Config::TagType;
+    static_assert(Ops::BITS == 8 * sizeof(Tag), "tags fill their lanes");
+    constexpr Word ones = cuckoo::detail::laneOnes<Tag, Word>();
+    constexpr Word highs = ones << (Ops::BITS - 1);
+    constexpr Word low = ~highs;
+    uint64_t e = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < Ops::WC; ++j) {
+        const Word x = s[j];
+        const Word zero = ~(((x & low) + low) | x | low);  // high bit of each zero lane, exactly
+#pragma unroll
+        for (uint32_t i = 0; i < Ops::TPW; ++i) {
+            e |= static_cast<uint64_t>((zero >> (i * Ops::BITS + Ops::BITS - 1)) & 1) << (j * Ops::TPW + i);
+        }
+    }
+    return e;
+}
 
 /// Primary-bucket attempts of one wave's inserts, shared by the lanes that
 /// insert into the same bucket.  Sorted packed tags put runs of consecutive
 /// lanes on one bucket (about 15 at 95% load with 16-slot buckets); one device
 /// CAS per lane on the same words serialises at the memory side.  Here the
 /// first lane of each run (consecutive active lanes with equal i1) reads the
-/// bucket once, places the run's tags one after another in lane order as
-/// tryInsertAtBucket would (CuckooFilter.cuh:859-904: first word from the
-/// tag's start word with an empty slot, its lowest empty slot), and writes
-/// each changed word with one CAS against the value it read.  That is the
-/// run's inserts done back to back, an order the reference's concurrent
-/// kernel allows.  Lanes exchange data through LDS, not cross-lane
-/// permutes.  Returns for the calling lane: kRunPlaced if its tag was
-/// stored, kRunFull if the bucket had no room for it (the rest of its insert
-/// starts at the secondary bucket), kRunRetry if a word it depended on
-/// changed under the run (its insert starts over from the primary bucket).
-
-template <typename Config>
+/// bucket once and places the run's tags one after another in lane order as
+/// tryInsertAtBucket would (CuckooFilter.cuh:859-904: the first word from the
+/// tag's start word with an empty slot, its lowest empty slot), which over a
+/// mask of the bucket's empty slots is the lowest set bit after rotating the
+/// mask by the start word: a few integer operations per tag.  Each lane then
+/// ORs its tag into the new value of its word (LDS atomics), and each changed
+/// word is written with one CAS against the value read, the c-th changed word
+/// by the run's c-th lane (a run of m tags changes at most m words): the
+/// run's CASes are one wave instruction.  That is the run's inserts done back
+/// to back, an order the reference's concurrent kernel allows.  Lanes
+/// exchange data through LDS and ballots, not cross-lane permutes.  Returns
+/// for the calling lane: kRunPlaced if its tag was stored, kRunFull if the
+/// bucket had no room for it (the rest of its insert starts at the secondary
+/// bucket), kRunRetry if a word it depended on changed under the run (its
+/// insert starts over from the primary bucket).
+template <typename Config, uint32_t Slots>
 __device__ __forceinline__ uint32_t primaryRunInsert(const DeviceView<Config>& v, bool active, size_t i1,
-                                                     typename Config::TagType fp, RunLds& lds) {
+                                                     typename Config::TagType fp, RunLds<Slots>& lds) {
     using Ops = FilterOps<Config>;
     using Word = typename Config::WordType;
-    using Tag = typename Config::TagType;
     constexpr uint32_t WC = Ops::WC;
-    constexpr uint8_t kFullWord = 0xff;  // "no word": the bucket was full
+    constexpr uint32_t TPW = Ops::TPW;
+    constexpr uint32_t BS = Ops::BS;
+    static_assert(BS <= 64, "one bit per slot in 64 bits");
+    constexpr uint64_t allSlots = BS == 64 ? ~0ull : ((1ull << BS) - 1);
+    constexpr uint64_t wordSlots = TPW == 64 ? ~0ull : ((1ull << TPW) - 1);
+    constexpr uint8_t kNone = 0xff;  // no slot (the bucket was full) / no word to write
     const uint32_t lane = __lane_id();
     const uint32_t tid = threadIdx.x;
     lds.bucket[tid] = i1;
@@ -301,65 +338,94 @@ __device__ __forceinline__ uint32_t primaryRunInsert(const DeviceView<Config>& v
     const bool head = active && (lane == 0 || lds.bucket[tid - 1] != static_cast<unsigned long long>(i1));
     const uint64_t act = __ballot(active);
     const uint64_t heads = __ballot(head);
+    const uint64_t upTo = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
+    waveSync();  // every lane has compared its neighbour's bucket before the slots take words
     if (head) {
-        const uint64_t upTo = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
         const uint64_t later = heads & ~upTo;
         const uint32_t runEnd = later != 0 ? static_cast<uint32_t>(__builtin_ctzll(later))
                                            : static_cast<uint32_t>(__popcll(act));
         const uint32_t len = runEnd - lane;
-        Word* bw = Ops::bucketPtr(v, i1);
-        const typename Ops::SnapT orig = loadBucket<Word, WC>(bw);
-        typename Ops::SnapT s = orig;
-        uint64_t dirty = 0;
+        const typename Ops::SnapT orig = loadBucket<Word, WC>(Ops::bucketPtr(v, i1));
+        const uint64_t empty0 = emptySlots<Config>(orig);
+        uint64_t empty = empty0;
         for (uint32_t k = 0; k < len; ++k) {
-            const Tag t = static_cast<Tag>(lds.tag[tid + k]);
-            uint8_t w = kFullWord;
-            const uint64_t m = Ops::emptyWords(s);
-            if (m != 0) {
-                w = static_cast<uint8_t>(firstFrom<WC>(m, Ops::startWord(t)));
-                const Word cur = pick<Word, WC>(s, w);
-                const uint32_t slot = lowestBit(getZeroMask<Tag, Word>(cur)) / Config::bitsPerTag;
-                s = put<Word, WC>(s, w, Ops::replaceTag(cur, slot, t));
-                dirty |= 1ull << w;
+            uint8_t slot = kNone;
+            if (empty != 0) {
+                const uint32_t sh = Ops::startWord(static_cast<typename Config::TagType>(lds.tag[tid + k])) * TPW;
+                uint64_t rot;
+                if constexpr (BS == 64) {
+                    rot = (empty >> sh) | (empty << ((64 - sh) & 63));
+                } else {
+                    rot = ((empty >> sh) | (empty << (BS - sh))) & allSlots;
+                }
+                const uint32_t sl = (sh + static_cast<uint32_t>(__builtin_ctzll(rot))) & (BS - 1);
+                empty &= ~(1ull << sl);
+                slot = static_cast<uint8_t>(sl);
             }
-            lds.out[tid + k] = w;
+            lds.out[tid + k] = slot;
         }
-        uint64_t failed = 0;
+        const uint64_t filled = empty0 & ~empty;
+        uint32_t dirty = 0;
+        uint32_t c = 0;
 #pragma unroll
         for (uint32_t j = 0; j < WC; ++j) {
-            if ((dirty >> j) & 1) {
+            if ((filled >> (j * TPW)) & wordSlots) {
+                dirty |= 1u << j;
                 const Word o = pick<Word, WC>(orig, j);
-                if (casRelaxed(bw + j, o, pick<Word, WC>(s, j)) != o) failed |= 1ull << j;
+                lds.bucket[tid + c] = static_cast<unsigned long long>(o);
+                lds.desired[tid + c] = static_cast<unsigned long long>(o);
+                lds.word[tid + c] = static_cast<uint8_t>(j);
+                ++c;
             }
         }
-        for (uint32_t k = 0; k < len; ++k) {
-            const uint8_t w = lds.out[tid + k];
-            uint8_t r;
-            if (w == kFullWord) {
-                r = failed == 0 ? kRunFull : kRunRetry;
-            } else {
-                r = ((failed >> w) & 1) ? kRunRetry : kRunPlaced;
-            }
-            lds.out[tid + k] = r;
+        for (; c < len; ++c) lds.word[tid + c] = kNone;
+        lds.tag[tid] = dirty;
+    }
+    waveSync();
+    // every lane of a run: its slot, the run's changed words, and the lane
+    // that writes its word (the run's first lane plus the word's rank)
+    uint32_t slot = kNone;
+    uint32_t headLane = 0;
+    uint32_t dirty = 0;
+    if (active) {
+        headLane = 63u - static_cast<uint32_t>(__builtin_clzll(heads & upTo));
+        dirty = lds.tag[tid - lane + headLane];
+        slot = lds.out[tid];
+        if (slot != kNone) {
+            const uint32_t w = slot / TPW;
+            const uint32_t rank = static_cast<uint32_t>(__popc(dirty & ((1u << w) - 1)));
+            const unsigned long long bits = static_cast<unsigned long long>(static_cast<Word>(fp))
+                                            << ((slot % TPW) * Ops::BITS);
+            __hip_atomic_fetch_or(&lds.desired[tid - lane + headLane + rank], bits, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WAVEFRONT);
         }
     }
     waveSync();
-    const uint32_t r = lds.out[tid];
-    waveSync();  // every lane has read its outcome before the slots are reused
-    return r;
+    const uint8_t myWord = active ? lds.word[tid] : kNone;
+    bool casFailed = false;
+    if (myWord != kNone) {
+        const Word o = static_cast<Word>(lds.bucket[tid]);
+        casFailed = casRelaxed(Ops::bucketPtr(v, i1) + myWord, o, static_cast<Word>(lds.desired[tid])) != o;
+    }
+    const uint64_t failedLanes = __ballot(casFailed);
+    if (!active) return kRunPlaced;  // not used
+    if (slot != kNone) {
+        const uint32_t rank = static_cast<uint32_t>(__popc(dirty & ((1u << (slot / TPW)) - 1)));
+        return ((failedLanes >> (headLane + rank)) & 1) ? kRunRetry : kRunPlaced;
+    }
+    const uint32_t writers = static_cast<uint32_t>(__popc(dirty));
+    const uint64_t runWriters = writers == 0 ? 0 : (((1ull << writers) - 1) << headLane);
+    return (failedLanes & runWriters) != 0 ? kRunRetry : kRunFull;
 }
 
-}  // namespace detail
-
-/// insertKernelSorted (CuckooFilter.cuh:1209-1266); `keys` is unused, as there.
-/// Buckets of up to 64 bytes take their primary attempts per run of lanes on
-/// one bucket (detail::primaryRunInsert); the secondary bucket and eviction
-/// follow per lane, as in the reference.  128-byte buckets keep one insert
-/// per lane: two register copies of such a bucket spill to scratch.
-template <typename Config>
-__global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename Config::KeyType* /*keys*/,
-                                                                const PackedTagOf<Config>* __restrict__ packedTags,
-                                                                size_t n, DeviceView<Config> v) {
+/// The body of insertKernelSorted with LDS for workgroups of up to Slots
+/// threads.  Buckets of up to 64 bytes take their primary attempts per run of
+/// lanes on one bucket (primaryRunInsert); the secondary bucket and eviction
+/// follow per lane, as in the reference.  128-byte buckets keep one insert per
+/// lane: two register copies of such a bucket spill to scratch.
+template <typename Config, uint32_t Slots>
+__device__ __forceinline__ void insertSortedBody(const PackedTagOf<Config>* __restrict__ packedTags, size_t n,
+                                                 const DeviceView<Config>& v) {
     using Ops = FilterOps<Config>;
     using Tag = typename Config::TagType;
     using P = PackedTagOf<Config>;
@@ -375,20 +441,24 @@ __global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename C
         }
     };
     if constexpr (Ops::kSnapshot && Ops::L::bucketBytes <= 64) {
-        __shared__ detail::RunLds runLds;
-        __shared__ detail::PendingLds pendingLds;
-        detail::PendingInserts<Config> pending(pendingLds);
+        __shared__ RunLds<Slots> runLds;
+        __shared__ PendingLds<Slots> pendingLds;
+        PendingInserts<Config, Slots> pending(pendingLds);
         // every lane of a wave runs the same iterations (the base is uniform
-        // per workgroup), so that the run's lanes meet in the wave operations
-        for (size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
-            const size_t i = base + threadIdx.x;
-            const bool active = i < n;
-            const P packed = active ? packedTags[i] : P(0);
+        // per workgroup), so that the run's lanes meet in the wave operations;
+        // the next iteration's packed tag is read ahead
+        size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x;
+        P packed = base + threadIdx.x < n ? packedTags[base + threadIdx.x] : P(0);
+        for (; base < n; base += stride) {
+            const bool active = base + threadIdx.x < n;
+            const size_t nextI = base + stride + threadIdx.x;
+            const P next = nextI < n ? packedTags[nextI] : P(0);
             const size_t i1 = static_cast<size_t>(packed >> Config::bitsPerTag);
             const Tag fp = static_cast<Tag>(packed & fpMask);
-            const uint32_t r = detail::primaryRunInsert<Config>(v, active, i1, fp, runLds);
-            if (active && r == detail::kRunPlaced) account(true, 0);
-            pending.push(v, active && r != detail::kRunPlaced, i1, fp, r, account);
+            const uint32_t r = primaryRunInsert<Config, Slots>(v, active, i1, fp, runLds);
+            if (active && r == kRunPlaced) account(true, 0);
+            pending.push(v, active && r != kRunPlaced, i1, fp, r, account);
+            packed = next;
         }
         pending.finish(v, account);
     } else {
@@ -400,8 +470,29 @@ __global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename C
             account(s, chain);
         }
     }
-    detail::blockCountAdd(ok, v.occupied, false);
-    if (v.evictions != nullptr) detail::blockCountAdd(ev, v.evictions, false);
+    blockCountAdd(ok, v.occupied, false);
+    if (v.evictions != nullptr) blockCountAdd(ev, v.evictions, false);
+}
+
+}  // namespace detail
+
+/// insertKernelSorted (CuckooFilter.cuh:1209-1266); `keys` is unused, as there.
+/// Any workgroup of up to 1024 threads.
+template <typename Config>
+__global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename Config::KeyType* /*keys*/,
+                                                                const PackedTagOf<Config>* __restrict__ packedTags,
+                                                                size_t n, DeviceView<Config> v) {
+    detail::insertSortedBody<Config, kMaxBlock>(packedTags, n, v);
+}
+
+/// insertKernelSorted for workgroups of at most kSortedSmallBlock threads: the
+/// same work with LDS sized for them (9 KiB per workgroup instead of 35 KiB),
+/// so that LDS does not bound how many workgroups a CU holds.
+inline constexpr uint32_t kSortedSmallBlock = 256;
+template <typename Config>
+__global__ __launch_bounds__(kSortedSmallBlock) void insertKernelSortedSmall(
+    const PackedTagOf<Config>* __restrict__ packedTags, size_t n, DeviceView<Config> v) {
+    detail::insertSortedBody<Config, kSortedSmallBlock>(packedTags, n, v);
 }
 
 /// Per-key (i1, i2, fp) exactly as getCandidateBuckets computes them.
