@@ -480,11 +480,17 @@ struct FilterOps {
     // eviction (CuckooFilter.cuh:917-1018)
     // ---------------------------------------------------------------------
 
-    /// insertWithEvictionDFS.  `chain` receives the number of swaps made.
-    static __device__ bool insertDFS(const View& v, Tag fp, size_t startBucket, uint32_t& chain) {
-        Tag curFp = fp;
-        size_t curB = startBucket;
-        for (uint32_t e = 0; e < v.maxEvictions; ++e) {
+    /// How a run of DFS swaps ended.
+    enum DfsState : uint32_t { kDfsPlaced = 0, kDfsExhausted = 1, kDfsPaused = 2 };
+
+    /// Swaps e, e+1, ... of insertWithEvictionDFS carrying curFp into curB:
+    /// ends when the carried tag lands (kDfsPlaced), when maxEvictions swaps
+    /// have been made (kDfsExhausted), or before swap number `limit`
+    /// (kDfsPaused).  curFp, curB and e then hold the state to resume from; e
+    /// counts the swaps made.
+    static __device__ uint32_t dfsSteps(const View& v, Tag& curFp, size_t& curB, uint32_t& e, uint32_t limit) {
+        for (; e < v.maxEvictions; ++e) {
+            if (e == limit) return kDfsPaused;
             const size_t slot = (static_cast<size_t>(curFp) + static_cast<size_t>(e) * 7) & (BS - 1);
             const uint32_t w = static_cast<uint32_t>(slot / TPW);
             const uint32_t lane = static_cast<uint32_t>(slot & (TPW - 1));
@@ -497,13 +503,26 @@ struct FilterOps {
                 if (old == cur) break;
                 cur = old;
             }
-            chain = e + 1;
             const auto alt = altWithNewFp(curB, evicted, v.numBuckets);
             curB = alt.bucket;
             curFp = alt.fp;
-            if (tryInsertAtBucket(v, curB, curFp)) return true;
+            if (tryInsertAtBucket(v, curB, curFp)) {
+                ++e;
+                return kDfsPlaced;
+            }
         }
-        return false;
+        return kDfsExhausted;
+    }
+
+    /// insertWithEvictionDFS (CuckooFilter.cuh:917-958).  `chain` receives
+    /// the number of swaps made.
+    static __device__ bool insertDFS(const View& v, Tag fp, size_t startBucket, uint32_t& chain) {
+        Tag curFp = fp;
+        size_t curB = startBucket;
+        uint32_t e = 0;
+        const bool placed = dfsSteps(v, curFp, curB, e, ~0u) == kDfsPlaced;
+        if (e != 0) chain = e;
+        return placed;
     }
 
     /// The BFS stage of insertWithEvictionBFS (CuckooFilter.cuh:968-1014): up to

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstddef>
 #include <cstdint>
 
@@ -66,8 +67,14 @@ __device__ __forceinline__ void waveSync() {
 
 /// How an insert's attempt at its primary bucket ended: stored; the bucket
 /// was full (the insert goes on at the secondary bucket, then eviction); or
-/// the bucket changed under the attempt (the insert starts over).
-enum RunOutcome : uint32_t { kRunPlaced = 0, kRunFull = 1, kRunRetry = 2 };
+/// the bucket changed under the attempt (the insert starts over).  A pending
+/// insert may also be a DFS chain paused between swaps (kRunResume).
+enum RunOutcome : uint32_t { kRunPlaced = 0, kRunFull = 1, kRunRetry = 2, kRunResume = 3 };
+
+/// DFS swaps a pending insert makes per round of its wave's queue (at 2^28
+/// slots, 95% load, DFS: 1, 2, 4 and 8 swaps per round took 5.26, 5.36, 5.60
+/// and 6.08 ms for the presorted insert of 255M keys).
+inline constexpr uint32_t kSwapsPerRound = 1;
 
 /// LDS slots of a workgroup's pending inserts: 64 per wave, at the wave's
 /// slots of the workgroup (Slots >= blockDim.x).
@@ -75,16 +82,22 @@ template <uint32_t Slots>
 struct PendingLds {
     unsigned long long bucket[Slots];
     uint32_t tag[Slots];
+    uint32_t swaps[Slots];  // a paused chain's swaps so far
     uint8_t kind[Slots];
 };
 
 /// One wave's inserts whose primary attempt did not finish.  They continue
 /// together, one per lane, once 64 have gathered (and at the end), instead of
-/// a few lanes at a time while the rest of the wave waits for them: an
-/// insert that goes on to eviction holds its wave for a chain of dependent
-/// bucket reads.  Continuing an insert later is an order the reference's
-/// concurrent kernel allows.  push() and finish() are called by every lane of
-/// the wave at the same points.
+/// a few lanes at a time while the rest of the wave waits for them.  An
+/// insert that goes on to DFS eviction makes at most kSwapsPerRound swaps per
+/// round and, if its chain is not done, goes back into the queue with its
+/// carried tag, bucket and swap count: a round costs the wave one swap
+/// rather than the longest of 64 chains.  Continuing an insert later, or
+/// pausing its chain between swaps, is an order the reference's concurrent
+/// kernel allows (its threads run at independent speeds); every chain still
+/// makes the reference's swaps in the reference's order, and finish() runs
+/// rounds until none is left.  push() and finish() are called by every lane
+/// of the wave at the same points.
 template <typename Config, uint32_t Slots>
 struct PendingInserts {
     using Ops = FilterOps<Config>;
@@ -106,11 +119,12 @@ struct PendingInserts {
                          Account& account) {
         const uint64_t deferred = __ballot(defer);
         const uint32_t count = static_cast<uint32_t>(__popcll(deferred));
-        if (queued + count > 64) drain(v, account);
+        while (queued + count > 64) round(v, account);
         if (defer) {
             const uint32_t slot = waveSlots + queued + static_cast<uint32_t>(__popcll(deferred & ((1ull << lane) - 1)));
             q.bucket[slot] = bucket;
             q.tag[slot] = tag;
+            q.swaps[slot] = 0;
             q.kind[slot] = static_cast<uint8_t>(kind);
         }
         queued += count;
@@ -118,26 +132,71 @@ struct PendingInserts {
 
     template <typename Account>
     __device__ void finish(const DeviceView<Config>& v, Account& account) {
-        if (queued != 0) drain(v, account);
+        while (queued != 0) round(v, account);
     }
 
    private:
+    /// One round over the queue: each entry's insert continues (the primary
+    /// bucket again for kRunRetry, the secondary bucket and BFS candidates
+    /// for kRunFull, as insertFrom and insertAfterPrimary go on) and then
+    /// makes up to kSwapsPerRound DFS swaps; chains not done are compacted to
+    /// the front of the queue.
     template <typename Account>
-    __device__ void drain(const DeviceView<Config>& v, Account& account) {
+    __device__ void round(const DeviceView<Config>& v, Account& account) {
         waveSync();
+        uint32_t kept = 0;  // wave-uniform
         for (uint32_t first = 0; first < queued; first += waveLanes) {
             const uint32_t k = first + lane;
+            bool keep = false;
+            size_t b = 0;
+            Tag t = 0;
+            uint32_t e = 0;
             if (k < queued) {
-                const size_t b = static_cast<size_t>(q.bucket[waveSlots + k]);
-                const Tag t = static_cast<Tag>(q.tag[waveSlots + k]);
+                b = static_cast<size_t>(q.bucket[waveSlots + k]);
+                t = static_cast<Tag>(q.tag[waveSlots + k]);
+                e = q.swaps[waveSlots + k];
+                const uint32_t kind = q.kind[waveSlots + k];
                 uint32_t chain = 0;
-                const bool s = q.kind[waveSlots + k] == kRunFull ? Ops::insertAfterPrimary(v, b, t, chain)
-                                                                 : Ops::insertFrom(v, b, t, chain);
-                account(s, chain);
+                bool placed = false;
+                if (kind == kRunRetry) placed = Ops::tryInsertAtBucket(v, b, t);
+                if (!placed && kind != kRunResume) {
+                    const auto second = Ops::altWithNewFp(b, t, v.numBuckets);
+                    placed = Ops::tryInsertAtBucket(v, second.bucket, second.fp);
+                    if (!placed) {
+                        const auto start = Ops::evictStart(b, t, second);
+                        if (v.evictionPolicy == static_cast<uint32_t>(EvictionPolicy::BFS)) {
+                            placed = Ops::bfsCandidates(v, start.fp, start.bucket, chain);
+                        }
+                        b = start.bucket;
+                        t = start.fp;
+                        e = 0;
+                    }
+                }
+                if (placed) {
+                    account(true, chain);
+                } else {
+                    const uint32_t limit = v.maxEvictions - e > kSwapsPerRound ? e + kSwapsPerRound : v.maxEvictions;
+                    const uint32_t st = Ops::dfsSteps(v, t, b, e, limit);
+                    if (st == Ops::kDfsPaused) {
+                        keep = true;
+                    } else {
+                        account(st == Ops::kDfsPlaced, e);
+                    }
+                }
             }
+            const uint64_t keeps = __ballot(keep);
+            waveSync();  // this pass's entries are read before paused chains are written back
+            if (keep) {
+                const uint32_t slot = waveSlots + kept + static_cast<uint32_t>(__popcll(keeps & ((1ull << lane) - 1)));
+                q.bucket[slot] = b;
+                q.tag[slot] = t;
+                q.swaps[slot] = e;
+                q.kind[slot] = static_cast<uint8_t>(kRunResume);
+            }
+            kept += static_cast<uint32_t>(__popcll(keeps));
         }
-        waveSync();  // the slots are free again
-        queued = 0;
+        waveSync();  // the queue holds the paused chains
+        queued = kept;
     }
 };
 
@@ -325,7 +384,10 @@ __device__ __forceinline__ uint32_t primaryRunInsert(const DeviceView<Config>& v
     constexpr uint32_t TPW = Ops::TPW;
     constexpr uint32_t BS = Ops::BS;
     static_assert(BS <= 64, "one bit per slot in 64 bits");
-    constexpr uint64_t allSlots = BS == 64 ? ~0ull : ((1ull << BS) - 1);
+    // slot masks in 32 bits where the bucket allows: half the shift and bit-scan work
+    using Mask = std::conditional_t<(BS <= 32), uint32_t, uint64_t>;
+    constexpr uint32_t kMaskBits = 8 * sizeof(Mask);
+    constexpr Mask allSlots = BS == kMaskBits ? static_cast<Mask>(~Mask(0)) : static_cast<Mask>((Mask(1) << BS) - 1);
     constexpr uint64_t wordSlots = TPW == 64 ? ~0ull : ((1ull << TPW) - 1);
     constexpr uint8_t kNone = 0xff;  // no slot (the bucket was full) / no word to write
     const uint32_t lane = __lane_id();
@@ -346,25 +408,32 @@ __device__ __forceinline__ uint32_t primaryRunInsert(const DeviceView<Config>& v
                                            : static_cast<uint32_t>(__popcll(act));
         const uint32_t len = runEnd - lane;
         const typename Ops::SnapT orig = loadBucket<Word, WC>(Ops::bucketPtr(v, i1));
-        const uint64_t empty0 = emptySlots<Config>(orig);
-        uint64_t empty = empty0;
+        const Mask empty0 = static_cast<Mask>(emptySlots<Config>(orig));
+        Mask empty = empty0;
+#pragma unroll 2
         for (uint32_t k = 0; k < len; ++k) {
             uint8_t slot = kNone;
             if (empty != 0) {
                 const uint32_t sh = Ops::startWord(static_cast<typename Config::TagType>(lds.tag[tid + k])) * TPW;
-                uint64_t rot;
-                if constexpr (BS == 64) {
-                    rot = (empty >> sh) | (empty << ((64 - sh) & 63));
+                Mask rot;
+                if constexpr (BS == kMaskBits) {
+                    rot = static_cast<Mask>((empty >> sh) | (empty << ((kMaskBits - sh) & (kMaskBits - 1))));
                 } else {
-                    rot = ((empty >> sh) | (empty << (BS - sh))) & allSlots;
+                    rot = static_cast<Mask>(((empty >> sh) | (empty << (BS - sh))) & allSlots);
                 }
-                const uint32_t sl = (sh + static_cast<uint32_t>(__builtin_ctzll(rot))) & (BS - 1);
-                empty &= ~(1ull << sl);
+                uint32_t low;
+                if constexpr (sizeof(Mask) == 4) {
+                    low = static_cast<uint32_t>(__builtin_ctz(rot));
+                } else {
+                    low = static_cast<uint32_t>(__builtin_ctzll(rot));
+                }
+                const uint32_t sl = (sh + low) & (BS - 1);
+                empty &= static_cast<Mask>(~(Mask(1) << sl));
                 slot = static_cast<uint8_t>(sl);
             }
             lds.out[tid + k] = slot;
         }
-        const uint64_t filled = empty0 & ~empty;
+        const uint64_t filled = static_cast<uint64_t>(empty0 & ~empty);
         uint32_t dirty = 0;
         uint32_t c = 0;
 #pragma unroll
@@ -486,7 +555,7 @@ __global__ __launch_bounds__(kMaxBlock) void insertKernelSorted(const typename C
 }
 
 /// insertKernelSorted for workgroups of at most kSortedSmallBlock threads: the
-/// same work with LDS sized for them (9 KiB per workgroup instead of 35 KiB),
+/// same work with LDS sized for them (10 KiB per workgroup instead of 39 KiB),
 /// so that LDS does not bound how many workgroups a CU holds.
 inline constexpr uint32_t kSortedSmallBlock = 256;
 template <typename Config>
