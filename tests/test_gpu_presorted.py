@@ -228,3 +228,39 @@ def test_presorted_duplicate_runs_keep_every_copy(cf, oracle_mod, eviction, bloc
     missing = int(np.maximum(reps - copies, 0).sum())
     assert missing <= n - occ, (missing, n - occ)
     f.close()
+
+
+# Inserts whose primary attempt fails continue in the wave's queue with one
+# DFS swap per round; a chain that is not done goes back into the queue with
+# its carried tag, bucket and swap count.  Short eviction bounds exhaust many
+# chains part-way through the queue.  Each insert must be accounted exactly
+# once: the histogram's failure bin equals the failed inserts, the swaps the
+# histogram records equal the eviction counter, and no placed key is missing.
+# The failure count stays in the band of the sequential oracle's over the same
+# sorted order (BFS 7682 / 5319 / 0, DFS 28130 / 15133 / 0 for 1 / 3 / 500).
+@pytest.mark.parametrize("block", [256, 1024])
+@pytest.mark.parametrize("max_ev", [1, 3, 500])
+@pytest.mark.parametrize("eviction", ["bfs", "dfs"])
+def test_presorted_paused_chains_account_every_insert(cf, oracle_mod, eviction, max_ev, block):
+    O = oracle_mod
+    c = cf.make_config(eviction=eviction, max_evictions=max_ev, block_size=block)
+    f = cf.CuckooFilter(1 << 20, config=c)
+    f.setEvictionHistogram(True)
+    n = int(f.capacity() * 0.98)
+    keys = O.gen_keys(83 + max_ev, n, c.key_bytes)
+    dk = cf.keys_to_device(keys)
+    occ = f.insertPackedSorted(f.sortPackedTags(f.computePackedTags(dk)))
+    failed = n - occ
+    hist = f.evictionHistogram().astype(np.int64)
+    assert int(hist[max_ev + 1]) == failed, (int(hist[max_ev + 1]), failed)
+    swaps = int((np.arange(1, max_ev + 1) * hist[1:max_ev + 1]).sum()) + max_ev * int(hist[max_ev + 1])
+    assert swaps == f.evictionCount(), (swaps, f.evictionCount())
+    assert_table_count(f, occ, n)
+    missing = int((~f.containsMany(dk)).sum().item())
+    assert missing <= failed, (missing, failed)
+    of = O.Filter(O.make_config(**c.as_dict()), 1 << 20)
+    want = n - of.insert_many_sorted(keys)
+    assert failed <= 2 * want + 100, (failed, want)
+    if want > 1000:
+        assert failed >= want // 2, (failed, want)
+    f.close()
