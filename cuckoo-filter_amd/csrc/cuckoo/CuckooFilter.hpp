@@ -78,6 +78,7 @@ using cuckoo::containsKernel;
 using cuckoo::deleteKernel;
 using cuckoo::insertKernel;
 using cuckoo::insertKernelSorted;
+using cuckoo::insertKernelSortedSmall;
 
 /**
  * @brief GPU cuckoo filter over one HIP device (CuckooFilter.cuh:139-1117).

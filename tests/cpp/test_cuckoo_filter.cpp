@@ -252,6 +252,14 @@ void presortedInsert(std::mt19937& rng) {
     check(inserted > n * 0.99, "presorted insert places at least 99% of the keys");
     check(filter->countOccupiedSlots() == inserted, "counter equals non-empty slots");
     check(countTrue(lookup(*filter, keys)) >= inserted, "presorted-inserted keys found");
+    // the variant for workgroups of up to 256 threads, launched the same way
+    filter->clear();
+    insertKernelSortedSmall<Config><<<numBlocks, Config::blockSize>>>(d_packedTags, n, filter.get());
+    HIP_CALL(hipDeviceSynchronize());
+    const size_t insertedSmall = filter->occupiedSlots();
+    check(insertedSmall > n * 0.99, "small-block presorted insert places at least 99% of the keys");
+    check(filter->countOccupiedSlots() == insertedSmall, "counter equals non-empty slots (small-block kernel)");
+    check(countTrue(lookup(*filter, keys)) >= insertedSmall, "small-block presorted-inserted keys found");
     HIP_CALL(hipFree(d_packedTags));
 }
 
