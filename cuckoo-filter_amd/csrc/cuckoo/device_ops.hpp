@@ -528,35 +528,38 @@ struct FilterOps {
     /// The BFS stage of insertWithEvictionBFS (CuckooFilter.cuh:968-1014): up to
     /// min(8, bucketSize/2) candidates at slots (fp + i*7) & (bucketSize-1);
     /// false means the caller continues with DFS from the same bucket (:1017).
+    static constexpr uint32_t kBfsCandidates = (BS / 2) < 8 ? uint32_t(BS / 2) : 8u;
+
     static __device__ bool bfsCandidates(const View& v, Tag fp, size_t startBucket, uint32_t& chain) {
-        constexpr uint32_t numCandidates = (BS / 2) < 8 ? uint32_t(BS / 2) : 8u;
-        if constexpr (numCandidates > 0) {
-            Word* sb = bucketPtr(v, startBucket);
-            for (uint32_t i = 0; i < numCandidates; ++i) {
-                const size_t slot = (static_cast<size_t>(fp) + static_cast<size_t>(i) * 7) & (BS - 1);
-                const uint32_t w = static_cast<uint32_t>(slot / TPW);
-                const uint32_t lane = static_cast<uint32_t>(slot & (TPW - 1));
-                Word cur = detail::atomicLoadRelaxed(sb + w);
-                const Tag cand = extractTag(cur, lane);
-                if (cand == EMPTY) {
-                    if (tryInsertAtBucket(v, startBucket, fp)) return true;
-                    continue;
+        for (uint32_t i = 0; i < kBfsCandidates; ++i) {
+            if (bfsCandidate(v, fp, startBucket, i, chain)) return true;
+        }
+        return false;
+    }
+
+    /// Candidate i of the BFS stage: true if the tag was placed (chain = 1
+    /// when a candidate moved to its alternate bucket to make room).
+    static __device__ bool bfsCandidate(const View& v, Tag fp, size_t startBucket, uint32_t i, uint32_t& chain) {
+        Word* sb = bucketPtr(v, startBucket);
+        const size_t slot = (static_cast<size_t>(fp) + static_cast<size_t>(i) * 7) & (BS - 1);
+        const uint32_t w = static_cast<uint32_t>(slot / TPW);
+        const uint32_t lane = static_cast<uint32_t>(slot & (TPW - 1));
+        Word cur = detail::atomicLoadRelaxed(sb + w);
+        const Tag cand = extractTag(cur, lane);
+        if (cand == EMPTY) return tryInsertAtBucket(v, startBucket, fp);
+        const auto alt = altWithNewFp(startBucket, cand, v.numBuckets);
+        if (tryInsertAtBucket(v, alt.bucket, alt.fp)) {
+            // swap our tag in only while the candidate still sits there
+            while (extractTag(cur, lane) == cand) {
+                const Word old = detail::casRelaxed(sb + w, cur, replaceTag(cur, lane, fp));
+                if (old == cur) {
+                    chain = 1;
+                    return true;
                 }
-                const auto alt = altWithNewFp(startBucket, cand, v.numBuckets);
-                if (tryInsertAtBucket(v, alt.bucket, alt.fp)) {
-                    // swap our tag in only while the candidate still sits there
-                    while (extractTag(cur, lane) == cand) {
-                        const Word old = detail::casRelaxed(sb + w, cur, replaceTag(cur, lane, fp));
-                        if (old == cur) {
-                            chain = 1;
-                            return true;
-                        }
-                        cur = old;
-                    }
-                    // :1011-1012 -- undo the copy placed in the alternate bucket
-                    tryRemoveAtBucket(v, alt.bucket, cand);
-                }
+                cur = old;
             }
+            // :1011-1012 -- undo the copy placed in the alternate bucket
+            tryRemoveAtBucket(v, alt.bucket, cand);
         }
         return false;
     }

@@ -69,7 +69,7 @@ __device__ __forceinline__ void waveSync() {
 /// was full (the insert goes on at the secondary bucket, then eviction); or
 /// the bucket changed under the attempt (the insert starts over).  A pending
 /// insert may also be a DFS chain paused between swaps (kRunResume).
-enum RunOutcome : uint32_t { kRunPlaced = 0, kRunFull = 1, kRunRetry = 2, kRunResume = 3 };
+enum RunOutcome : uint32_t { kRunPlaced = 0, kRunFull = 1, kRunRetry = 2, kRunResume = 3, kRunBfs = 4 };
 
 /// DFS swaps a pending insert makes per round of its wave's queue (at 2^28
 /// slots, 95% load, DFS: 1, 2, 4 and 8 swaps per round took 5.26, 5.36, 5.60
@@ -89,10 +89,11 @@ struct PendingLds {
 /// One wave's inserts whose primary attempt did not finish.  They continue
 /// together, one per lane, once 64 have gathered (and at the end), instead of
 /// a few lanes at a time while the rest of the wave waits for them.  An
-/// insert that goes on to DFS eviction makes at most kSwapsPerRound swaps per
-/// round and, if its chain is not done, goes back into the queue with its
-/// carried tag, bucket and swap count: a round costs the wave one swap
-/// rather than the longest of 64 chains.  Continuing an insert later, or
+/// insert that goes on to eviction tries one BFS candidate (BFS policy) or
+/// makes at most kSwapsPerRound DFS swaps per round and, if not done, goes
+/// back into the queue with its tag, bucket and candidate index or swap
+/// count: a round costs the wave one step rather than the longest of 64
+/// searches.  Continuing an insert later, or
 /// pausing its chain between swaps, is an order the reference's concurrent
 /// kernel allows (its threads run at independent speeds); every chain still
 /// makes the reference's swaps in the reference's order, and finish() runs
@@ -136,11 +137,13 @@ struct PendingInserts {
     }
 
    private:
-    /// One round over the queue: each entry's insert continues (the primary
-    /// bucket again for kRunRetry, the secondary bucket and BFS candidates
-    /// for kRunFull, as insertFrom and insertAfterPrimary go on) and then
-    /// makes up to kSwapsPerRound DFS swaps; chains not done are compacted to
-    /// the front of the queue.
+    /// One round over the queue: each entry's insert continues as insertFrom
+    /// and insertAfterPrimary go on (the primary bucket again for kRunRetry,
+    /// then the secondary bucket), then takes one step of its eviction: the
+    /// next BFS candidate (kRunBfs, the candidate index in `swaps`; DFS from
+    /// the same start once all have failed) or up to kSwapsPerRound DFS swaps
+    /// (kRunResume).  Entries not done are compacted to the front of the
+    /// queue.
     template <typename Account>
     __device__ void round(const DeviceView<Config>& v, Account& account) {
         waveSync();
@@ -151,30 +154,36 @@ struct PendingInserts {
             size_t b = 0;
             Tag t = 0;
             uint32_t e = 0;
+            uint32_t kind = kRunResume;
             if (k < queued) {
                 b = static_cast<size_t>(q.bucket[waveSlots + k]);
                 t = static_cast<Tag>(q.tag[waveSlots + k]);
                 e = q.swaps[waveSlots + k];
-                const uint32_t kind = q.kind[waveSlots + k];
+                kind = q.kind[waveSlots + k];
                 uint32_t chain = 0;
                 bool placed = false;
                 if (kind == kRunRetry) placed = Ops::tryInsertAtBucket(v, b, t);
-                if (!placed && kind != kRunResume) {
+                if (!placed && (kind == kRunRetry || kind == kRunFull)) {
                     const auto second = Ops::altWithNewFp(b, t, v.numBuckets);
                     placed = Ops::tryInsertAtBucket(v, second.bucket, second.fp);
                     if (!placed) {
                         const auto start = Ops::evictStart(b, t, second);
-                        if (v.evictionPolicy == static_cast<uint32_t>(EvictionPolicy::BFS)) {
-                            placed = Ops::bfsCandidates(v, start.fp, start.bucket, chain);
-                        }
                         b = start.bucket;
                         t = start.fp;
                         e = 0;
+                        kind = v.evictionPolicy == static_cast<uint32_t>(EvictionPolicy::BFS) ? kRunBfs : kRunResume;
                     }
                 }
-                if (placed) {
-                    account(true, chain);
-                } else {
+                if (!placed && kind == kRunBfs) {
+                    if (e < Ops::kBfsCandidates) placed = Ops::bfsCandidate(v, t, b, e++, chain);
+                    if (!placed) {
+                        keep = true;
+                        if (e >= Ops::kBfsCandidates) {  // DFS from the same start next round
+                            kind = kRunResume;
+                            e = 0;
+                        }
+                    }
+                } else if (!placed) {
                     const uint32_t limit = v.maxEvictions - e > kSwapsPerRound ? e + kSwapsPerRound : v.maxEvictions;
                     const uint32_t st = Ops::dfsSteps(v, t, b, e, limit);
                     if (st == Ops::kDfsPaused) {
@@ -183,6 +192,7 @@ struct PendingInserts {
                         account(st == Ops::kDfsPlaced, e);
                     }
                 }
+                if (placed) account(true, chain);
             }
             const uint64_t keeps = __ballot(keep);
             waveSync();  // this pass's entries are read before paused chains are written back
@@ -191,7 +201,7 @@ struct PendingInserts {
                 q.bucket[slot] = b;
                 q.tag[slot] = t;
                 q.swaps[slot] = e;
-                q.kind[slot] = static_cast<uint8_t>(kRunResume);
+                q.kind[slot] = static_cast<uint8_t>(kind);
             }
             kept += static_cast<uint32_t>(__popcll(keeps));
         }
