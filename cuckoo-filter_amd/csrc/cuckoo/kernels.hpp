@@ -68,7 +68,8 @@ __device__ __forceinline__ void waveSync() {
 /// How an insert's attempt at its primary bucket ended: stored; the bucket
 /// was full (the insert goes on at the secondary bucket, then eviction); or
 /// the bucket changed under the attempt (the insert starts over).  A pending
-/// insert may also be a DFS chain paused between swaps (kRunResume).
+/// insert may also be a BFS search paused between candidates (kRunBfs) or a
+/// DFS chain paused between swaps (kRunResume).
 enum RunOutcome : uint32_t { kRunPlaced = 0, kRunFull = 1, kRunRetry = 2, kRunResume = 3, kRunBfs = 4 };
 
 /// DFS swaps a pending insert makes per round of its wave's queue (at 2^28
